@@ -1,0 +1,262 @@
+"""Topology headers and model tables for the HIP step.
+
+A System splits into two parts, the same split the reference makes between
+its static (non-pytree) fields and its arrays (brax/base.py:471-480):
+
+* the *topology*: link types and parents, q/qd addresses, actuator and limit
+  dof maps, and the static list of contact candidates (link + geom kind).
+  It is compiled into the kernel as a ``constexpr`` struct, so every
+  per-link and per-dof array of the step lives in registers with static
+  indices.  ``write_topology_header`` emits these structs for the registered
+  systems; ``build()`` regenerates them before compiling.
+* the *model table*: every number the step reads (link frames, inertias,
+  dof axes, limits, solver parameters, actuator gains, geom sizes and
+  contact parameters), packed by ``pack_model`` into one float32 array with
+  the fixed record layout below, uploaded once per model.
+"""
+
+from __future__ import annotations
+
+import hashlib
+import os
+from typing import Dict, List, Tuple
+
+import numpy as np
+
+from brax_amd import systems as S
+
+# record layouts of the float32 model table (mirrored in csrc/bgx_layout.h)
+HEADER = 8  # gravity[3], dt, solver_iterations, solver_maxls, tol, ls_eps
+LINK_STRIDE = 24  # tpos[3] trot[4] jpos[3] ipos[3] irot[4] I[6] mass
+DOF_STRIDE = 20  # ang[3] vel[3] armature damping stiffness lo hi invweight solref[2] solimp[5]
+ACT_STRIDE = 8  # ctrl_lo ctrl_hi force_lo force_hi gain gear bias_q bias_qd
+CAND_STRIDE = 32  # gpos[3] gquat[4] radius halflen sign mu solref[2] solimp[5] tw ppos[3] pn[3] t1[3] t2[3]
+
+KIND_SPHERE = 0
+KIND_CAPSULE = 1
+
+# fast-path contact-slot capacity per topology (other candidates take the
+# scratch path); sized from the active-contact distribution of the workload
+DEFAULT_SLOTS = 6
+
+JAXOPT_TOL = 1e-3  # jaxopt ProximalGradient default tol
+LS_EPS = float(np.finfo(np.float32).eps)
+
+
+class Topology:
+  """Static structure of a System, as consumed by the kernel templates."""
+
+  def __init__(self, sys: S.System, slots: int = DEFAULT_SLOTS):
+    S.from_arrays_check(sys)
+    self.name = _ident(sys.name)
+    self.link_types = sys.link_types
+    self.parents = list(sys.link_parents)
+    self.L = sys.num_links()
+    self.nq, self.nv, self.nu = sys.q_size(), sys.qd_size(), sys.act_size()
+    self.ltype = [0 if t == "f" else int(t) for t in sys.link_types]
+    self.qadr = [r[0] for r in sys.q_ranges()]
+    self.dadr = [r[0] for r in sys.dof_ranges()]
+    roots = []
+    for i in range(self.L):
+      r = i
+      while self.parents[r] >= 0:
+        r = self.parents[r]
+      roots.append(r)
+    self.root = roots
+    self.dof_link = [int(v) for v in sys.dof_link()]
+    # previous dof on the path to the root (-1 at a root link's first dof)
+    dp = []
+    for l, dr in enumerate(sys.dof_ranges()):
+      for k, d in enumerate(dr):
+        if k > 0:
+          dp.append(d - 1)
+        else:
+          p = self.parents[l]
+          dp.append(sys.dof_ranges()[p][-1] if p >= 0 else -1)
+    self.dof_parent = dp
+    if self.nv > 32:
+      raise NotImplementedError("at most 32 dofs per tree")
+    anc = []
+    for d in range(self.nv):
+      m, j = 0, dp[d]
+      while j >= 0:
+        m |= 1 << j
+        j = dp[j]
+      anc.append(m)
+    self.dof_anc = anc  # bit j set: dof j is a strict ancestor of dof d
+    self.act_q = [int(v) for v in sys.actuator.q_id]
+    self.act_d = [int(v) for v in sys.actuator.qd_id]
+    if sys.dof.limit is None:
+      self.lim_d, self.lim_q = [], []
+    else:
+      self.lim_d = [int(v) for v in sys.qd_idx("123")]
+      self.lim_q = [int(v) for v in sys.q_idx("123")]
+    self.cand_link, self.cand_kind, self.cand_src = [], [], []
+    for p in sys.contact_pairs():
+      la = int(sys.geom_bodyid[p.geom1]) - 1
+      lb = int(sys.geom_bodyid[p.geom2]) - 1
+      if la != -1 or lb < 0:
+        raise NotImplementedError("planes must belong to the world body")
+      kinds = ([(KIND_SPHERE, 0.0)] if p.kind == S.PAIR_PLANE_SPHERE
+               else [(KIND_CAPSULE, 1.0), (KIND_CAPSULE, -1.0)])
+      for kind, sign in kinds:
+        self.cand_link.append(lb)
+        self.cand_kind.append(kind)
+        self.cand_src.append((p, sign))
+    self.ncand = len(self.cand_link)
+    self.slots = max(1, min(slots, self.ncand)) if self.ncand else 1
+    self.off_link = HEADER
+    self.off_dof = self.off_link + LINK_STRIDE * self.L
+    self.off_act = self.off_dof + DOF_STRIDE * self.nv
+    self.off_cand = self.off_act + ACT_STRIDE * max(self.nu, 1)
+    self.size = self.off_cand + CAND_STRIDE * max(self.ncand, 1)
+
+  def signature(self) -> str:
+    key = repr((self.link_types, self.parents, self.act_q, self.act_d,
+                self.lim_d, self.cand_link, self.cand_kind, self.slots))
+    return hashlib.sha1(key.encode()).hexdigest()[:16]
+
+  def cpp_struct(self) -> str:
+    def arr(name, vals):
+      vals = list(vals) if vals else [0]
+      return (f"  static constexpr int {name}[{len(vals)}] = "
+              f"{{{', '.join(str(int(v)) for v in vals)}}};\n")
+    s = f"struct Topo_{self.name} {{\n"
+    s += f'  static constexpr const char* NAME = "{self.name}";\n'
+    s += (f"  static constexpr int L = {self.L}, NQ = {self.nq}, NV = {self.nv}, "
+          f"NU = {self.nu}, NLIM = {len(self.lim_d)}, NCAND = {self.ncand}, "
+          f"KC = {self.slots};\n")
+    s += (f"  static constexpr int OFF_LINK = {self.off_link}, OFF_DOF = "
+          f"{self.off_dof}, OFF_ACT = {self.off_act}, OFF_CAND = "
+          f"{self.off_cand}, SIZE = {self.size};\n")
+    s += arr("LTYPE", self.ltype) + arr("PARENT", self.parents)
+    s += arr("QADR", self.qadr) + arr("DADR", self.dadr) + arr("ROOT", self.root)
+    s += arr("DOF_LINK", self.dof_link) + arr("DOF_PARENT", self.dof_parent)
+    s += (f"  static constexpr unsigned DANC[{self.nv}] = "
+          f"{{{', '.join(hex(v) + 'u' for v in self.dof_anc)}}};\n")
+    s += arr("ACT_Q", self.act_q) + arr("ACT_D", self.act_d)
+    s += arr("LIM_D", self.lim_d) + arr("LIM_Q", self.lim_q)
+    s += arr("CAND_LINK", self.cand_link) + arr("CAND_KIND", self.cand_kind)
+    s += f'  static constexpr const char* SIG = "{self.signature()}";\n'
+    s += "};\n"
+    return s
+
+
+def _ident(name: str) -> str:
+  out = "".join(c if c.isalnum() else "_" for c in name)
+  return out if out and not out[0].isdigit() else "t_" + out
+
+
+def pack_model(sys: S.System, topo: Topology = None) -> np.ndarray:
+  """float32 model table of ``sys`` in the layout of its topology."""
+  topo = topo or Topology(sys)
+  m = np.zeros(topo.size, np.float64)
+  m[0:3] = sys.gravity
+  m[3] = sys.opt.timestep
+  m[4] = sys.solver_iterations
+  m[5] = sys.solver_maxls
+  m[6] = JAXOPT_TOL
+  m[7] = LS_EPS
+  for l in range(topo.L):
+    o = topo.off_link + l * LINK_STRIDE
+    m[o:o + 3] = sys.link.transform.pos[l]
+    m[o + 3:o + 7] = sys.link.transform.rot[l]
+    m[o + 7:o + 10] = sys.link.joint.pos[l]
+    m[o + 10:o + 13] = sys.link.inertia.transform.pos[l]
+    m[o + 13:o + 17] = sys.link.inertia.transform.rot[l]
+    i = sys.link.inertia.i[l]
+    m[o + 17:o + 23] = [i[0, 0], i[1, 1], i[2, 2], i[0, 1], i[0, 2], i[1, 2]]
+    m[o + 23] = sys.link.inertia.mass[l]
+  lo = hi = None
+  if sys.dof.limit is not None:
+    lo, hi = sys.dof.limit
+  for d in range(topo.nv):
+    o = topo.off_dof + d * DOF_STRIDE
+    m[o:o + 3] = sys.dof.motion.ang[d]
+    m[o + 3:o + 6] = sys.dof.motion.vel[d]
+    m[o + 6] = sys.dof.armature[d]
+    m[o + 7] = sys.dof.damping[d]
+    m[o + 8] = sys.dof.stiffness[d]
+    m[o + 9] = -np.inf if lo is None else lo[d]
+    m[o + 10] = np.inf if hi is None else hi[d]
+    m[o + 11] = sys.dof.invweight[d]
+    m[o + 12:o + 19] = sys.dof.solver_params[d]
+  a = sys.actuator
+  for u in range(topo.nu):
+    o = topo.off_act + u * ACT_STRIDE
+    m[o:o + 8] = [a.ctrl_range[u, 0], a.ctrl_range[u, 1], a.force_range[u, 0],
+                  a.force_range[u, 1], a.gain[u], a.gear[u], a.bias_q[u],
+                  a.bias_qd[u]]
+  for c in range(topo.ncand):
+    p, sign = topo.cand_src[c]
+    o = topo.off_cand + c * CAND_STRIDE
+    g2, g1 = p.geom2, p.geom1
+    m[o:o + 3] = sys.geom_pos[g2]
+    m[o + 3:o + 7] = sys.geom_quat[g2]
+    m[o + 7] = sys.geom_size[g2][0]
+    m[o + 8] = sys.geom_size[g2][1]
+    m[o + 9] = sign
+    m[o + 10] = p.friction[0]
+    m[o + 11:o + 13] = p.solref
+    m[o + 13:o + 18] = p.solimp
+    lb = int(sys.geom_bodyid[g2]) - 1
+    m[o + 18] = sys.link.invweight[lb]  # world side contributes 0
+    # plane on the world body: pose = geom pose (contact.py:48-53)
+    pr = S.quat_to_mat(sys.geom_quat[g1])
+    n = pr[:, 2]
+    m[o + 19:o + 22] = sys.geom_pos[g1]
+    m[o + 22:o + 25] = n
+    # sphere frame: [n, b, n x b] with b = orthogonals(n) (math.py:247-254)
+    nn = n / np.linalg.norm(n)
+    b = np.array([0.0, 1.0, 0.0]) if -0.5 < nn[1] < 0.5 else np.array([0.0, 0.0, 1.0])
+    b = b - nn * nn.dot(b)
+    b = b / np.linalg.norm(b)
+    m[o + 25:o + 28] = b
+    m[o + 28:o + 31] = np.cross(nn, b)
+  return m.astype(np.float32)
+
+
+# ---------------------------------------------------------------------------
+# registry of topologies compiled into the library
+
+
+def registered_systems() -> List[S.System]:
+  """Systems whose topologies are compiled into libbrax_gen.so."""
+  return [
+      S.quadruped(0),
+      S.ant(),
+      S.pendulum(1, motor_gear=150.0),
+      S.pendulum(3),
+  ]
+
+
+def registered_topologies() -> List[Topology]:
+  seen: Dict[str, Topology] = {}
+  for sys in registered_systems():
+    t = Topology(sys)
+    seen.setdefault(t.signature(), t)
+  return list(seen.values())
+
+
+def write_topology_header(path: str) -> Tuple[str, List[Topology]]:
+  topos = registered_topologies()
+  names = set()
+  out = ["// Generated by brax_amd/codegen.py from the registered systems.",
+         "// Static structure of each topology compiled into the library.",
+         "#pragma once", ""]
+  for t in topos:
+    if t.name in names:
+      t.name = f"{t.name}_{t.signature()[:6]}"
+    names.add(t.name)
+    out.append(t.cpp_struct())
+  out.append("#define BGX_FOR_EACH_TOPOLOGY(X) \\")
+  for i, t in enumerate(topos):
+    out.append(f"  X({i}, Topo_{t.name}) \\")
+  out.append("")
+  out.append(f"#define BGX_NUM_TOPOLOGIES {len(topos)}")
+  text = "\n".join(out) + "\n"
+  old = open(path).read() if os.path.exists(path) else None
+  if old != text:
+    with open(path, "w") as f:
+      f.write(text)
+  return text, topos

@@ -1,0 +1,215 @@
+"""Generalized-coordinates physics pipeline on MI355X.
+
+Mirrors brax.generalized.pipeline (brax/generalized/pipeline.py:32-94): the
+same ``init(sys, q, qd, ...)`` / ``step(sys, state, act, ...)`` signatures and
+State layout, batched over a leading env axis.  Every call runs hand-written
+HIP kernels from libbrax_gen.so through its C ABI (include/brax_gen.h); there
+is no host or PyTorch fallback, and a missing library raises.
+
+Also provided, for the callers of the reference's step:
+  * ``pipeline_step``: n_frames substeps with one action
+    (PipelineEnv.pipeline_step, brax/envs/base.py:128-137);
+  * ``rollout``: n_steps env steps in one launch, optionally returning the
+    (q, qd) trajectory.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from typing import Dict, Optional, Tuple
+
+import numpy as np
+import torch
+
+from brax_amd import _lib
+from brax_amd import codegen
+from brax_amd import systems as S
+from brax_amd.generalized.base import Motion, State, Transform
+
+STATUS_SLOW_PATH = 1
+STATUS_NONFINITE = 2
+
+
+class DeviceModel:
+  """A System uploaded to one GPU: topology id + device model table."""
+
+  def __init__(self, sys: S.System, device: torch.device):
+    self.sys = sys
+    self.device = device
+    topo = codegen.Topology(sys)
+    compiled = _lib.topologies()
+    if topo.signature() not in compiled:
+      raise NotImplementedError(
+          f"the topology of system {sys.name!r} is not compiled into "
+          "libbrax_gen.so; add it to brax_amd.codegen.registered_systems() "
+          "and rebuild")
+    self.topo_id, _, dims = compiled[topo.signature()]
+    self.L, self.nq, self.nv, self.nu = dims[0], dims[1], dims[2], dims[3]
+    table = np.ascontiguousarray(codegen.pack_model(sys, topo))
+    handle = ctypes.c_void_p()
+    lib = _lib.load()
+    with torch.cuda.device(device):
+      _lib.check(lib.bgx_model_create(self.topo_id, table.ctypes.data,
+                                      table.size, ctypes.byref(handle)))
+    self.handle = handle
+
+  def __del__(self):
+    h = getattr(self, "handle", None)
+    if h is not None and h.value:
+      try:
+        _lib.load().bgx_model_destroy(h)
+      except Exception:  # interpreter shutdown
+        pass
+
+
+_MODELS: Dict[Tuple[int, str], Tuple[S.System, DeviceModel]] = {}
+
+
+def device_model(sys: S.System, device=None) -> DeviceModel:
+  device = torch.device(device if device is not None else "cuda")
+  if device.type != "cuda":
+    raise ValueError("the generalized pipeline runs on the GPU only")
+  if device.index is None:
+    device = torch.device("cuda", torch.cuda.current_device())
+  key = (id(sys), str(device))
+  hit = _MODELS.get(key)
+  if hit is None or hit[0] is not sys:
+    S.from_arrays_check(sys)
+    hit = (sys, DeviceModel(sys, device))
+    _MODELS[key] = hit
+  return hit[1]
+
+
+def _as_batch(x, n_cols: int, device, name: str) -> torch.Tensor:
+  t = torch.as_tensor(x, dtype=torch.float32, device=device)
+  if t.dim() == 1:
+    t = t.unsqueeze(0)
+  if t.dim() != 2 or t.shape[1] != n_cols:
+    raise ValueError(f"{name} must have shape (N, {n_cols}), got {tuple(t.shape)}")
+  return t.contiguous()
+
+
+def _stream():
+  return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _frames(m: DeviceModel, n: int, device):
+  return (torch.empty(n, m.L, 3, device=device),
+          torch.empty(n, m.L, 4, device=device),
+          torch.empty(n, m.L, 3, device=device),
+          torch.empty(n, m.L, 3, device=device))
+
+
+def _debug_unsupported(debug: bool):
+  if debug:
+    raise NotImplementedError("debug contact output is not produced by this step")
+
+
+def init(sys: S.System, q, qd, unused_act=None, unused_ctrl=None,
+         debug: bool = False, device=None) -> State:
+  """pipeline.init (brax/generalized/pipeline.py:32-61)."""
+  del unused_act, unused_ctrl
+  _debug_unsupported(debug)
+  if device is None:
+    device = q.device if torch.is_tensor(q) and q.is_cuda else "cuda"
+  m = device_model(sys, device)
+  qb = _as_batch(q, m.nq, m.device, "q").clone()
+  qdb = _as_batch(qd, m.nv, m.device, "qd").clone()
+  n = qb.shape[0]
+  xp, xr, xa, xv = _frames(m, n, m.device)
+  lib = _lib.load()
+  _lib.check(lib.bgx_init(m.handle, n, _lib.ptr(qb), _lib.ptr(qdb), _lib.ptr(xp),
+                          _lib.ptr(xr), _lib.ptr(xa), _lib.ptr(xv), _stream()))
+  return State(q=qb, qd=qdb, x=Transform(xp, xr), xd=Motion(xa, xv))
+
+
+def _act(m: DeviceModel, act, n: int):
+  if act is None or m.nu == 0:
+    return None
+  a = _as_batch(act, m.nu, m.device, "act")
+  if a.shape[0] == 1 and n > 1:
+    a = a.expand(n, m.nu).contiguous()
+  if a.shape[0] != n:
+    raise ValueError(f"act has {a.shape[0]} rows for {n} envs")
+  return a
+
+
+def pipeline_step(sys: S.System, state: State, act, n_frames: int = 1,
+                  debug: bool = False) -> State:
+  """``n_frames`` substeps with one action (brax/envs/base.py:128-137)."""
+  _debug_unsupported(debug)
+  m = device_model(sys, state.q.device)
+  q = state.q.contiguous().clone()
+  qd = state.qd.contiguous().clone()
+  n = q.shape[0]
+  a = _act(m, act, n)
+  xp, xr, xa, xv = _frames(m, n, m.device)
+  status = torch.zeros(n, dtype=torch.int32, device=m.device)
+  lib = _lib.load()
+  _lib.check(lib.bgx_step(m.handle, n, int(n_frames), _lib.ptr(q), _lib.ptr(qd),
+                          _lib.ptr(a), _lib.ptr(xp), _lib.ptr(xr), _lib.ptr(xa),
+                          _lib.ptr(xv), _lib.ptr(status), _stream()))
+  return State(q=q, qd=qd, x=Transform(xp, xr), xd=Motion(xa, xv),
+               status=status)
+
+
+def step(sys: S.System, state: State, act, debug: bool = False) -> State:
+  """pipeline.step (brax/generalized/pipeline.py:64-94): one substep."""
+  return pipeline_step(sys, state, act, 1, debug)
+
+
+def rollout(sys: S.System, state: State, acts, n_frames: int,
+            trajectory: bool = False, frames: bool = True,
+            stream=None) -> Tuple[State, Optional[Tuple[torch.Tensor, torch.Tensor]]]:
+  """``acts.shape[0]`` env steps of ``n_frames`` substeps in one launch.
+
+  acts: (T, N, act_size) float32 on the GPU.  Returns the final State and,
+  with ``trajectory=True``, the (q, qd) after every env step as tensors of
+  shape (T, N, q_size) and (T, N, qd_size).
+  """
+  m = device_model(sys, state.q.device)
+  q = state.q.contiguous().clone()
+  qd = state.qd.contiguous().clone()
+  n = q.shape[0]
+  acts = torch.as_tensor(acts, dtype=torch.float32, device=m.device)
+  if acts.dim() != 3 or acts.shape[1] != n or acts.shape[2] != m.nu:
+    raise ValueError(f"acts must have shape (T, {n}, {m.nu})")
+  acts = acts.contiguous()
+  T = acts.shape[0]
+  qt = torch.empty(T, n, m.nq, device=m.device) if trajectory else None
+  qdt = torch.empty(T, n, m.nv, device=m.device) if trajectory else None
+  xp = xr = xa = xv = None
+  if frames:
+    xp, xr, xa, xv = _frames(m, n, m.device)
+  status = torch.zeros(n, dtype=torch.int32, device=m.device)
+  lib = _lib.load()
+  s = ctypes.c_void_p(stream) if stream is not None else _stream()
+  _lib.check(lib.bgx_rollout(m.handle, n, T, int(n_frames), _lib.ptr(q),
+                             _lib.ptr(qd), _lib.ptr(acts) if m.nu else None,
+                             n * m.nu, _lib.ptr(qt), _lib.ptr(qdt),
+                             _lib.ptr(xp), _lib.ptr(xr), _lib.ptr(xa),
+                             _lib.ptr(xv), _lib.ptr(status), s))
+  st = State(q=q, qd=qd,
+             x=Transform(xp, xr) if frames else None,
+             xd=Motion(xa, xv) if frames else None, status=status)
+  return st, ((qt, qdt) if trajectory else None)
+
+
+def probe(sys: S.System, state: State, act, n_frames: int = 1) -> Dict[str, torch.Tensor]:
+  """Diagnostics of the last substep: qf_smooth, qdd, active contacts,
+  solver iterations, line-search trials (bgx_step_probe)."""
+  m = device_model(sys, state.q.device)
+  q = state.q.contiguous().clone()
+  qd = state.qd.contiguous().clone()
+  n = q.shape[0]
+  a = _act(m, act, n)
+  w = 2 * m.nv + 3
+  out = torch.zeros(n, w, device=m.device)
+  lib = _lib.load()
+  _lib.check(lib.bgx_step_probe(m.handle, n, int(n_frames), _lib.ptr(q),
+                                _lib.ptr(qd), _lib.ptr(a), _lib.ptr(out),
+                                _stream()))
+  nv = m.nv
+  return {"q": q, "qd": qd, "qf_smooth": out[:, :nv], "qdd": out[:, nv:2 * nv],
+          "n_contacts": out[:, 2 * nv], "iters": out[:, 2 * nv + 1],
+          "trials": out[:, 2 * nv + 2]}

@@ -1,0 +1,110 @@
+/* brax_gen.h -- C ABI of the MI355X generalized-coordinates physics step.
+ *
+ * The drop-in boundary for brax's generalized pipeline.  The reference's
+ * interface is a Python module with two functions,
+ *   init(sys, q, qd, unused_act=None, unused_ctrl=None, debug=False) -> State
+ *     (brax/generalized/pipeline.py:32-61)
+ *   step(sys, state, act, debug=False) -> State
+ *     (brax/generalized/pipeline.py:64-94)
+ * driven n_frames times per env step by PipelineEnv.pipeline_step
+ * (brax/envs/base.py:128-137) and batched by jax.vmap
+ * (brax/envs/wrappers/training.py:59-72).  These entry points are the batched
+ * equivalents; brax_amd/generalized/pipeline.py binds them with ctypes.
+ *
+ * Conventions: every array is a device pointer owned by the caller, float32,
+ * env-major ("(N, NQ)" means env e's q at q + e * NQ).  Calls are enqueued on
+ * the given HIP stream (NULL = default stream) and return at once; a nonzero
+ * return is an error code and bgx_last_error() describes it.  The library
+ * holds no per-call workspace and no mutable global state beyond the
+ * per-thread error string, so calls on different streams may overlap.
+ */
+#ifndef BRAX_GEN_H_
+#define BRAX_GEN_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  BGX_OK = 0,
+  BGX_ERR_ARG = 1,      /* null pointer, bad size or count */
+  BGX_ERR_TOPOLOGY = 2, /* topology id not compiled into this library */
+  BGX_ERR_HIP = 3,      /* HIP runtime error (allocation, launch) */
+};
+
+/* per-env status bits written by bgx_step / bgx_rollout */
+enum {
+  BGX_STATUS_SLOW_PATH = 1, /* more active contacts than the fast slots */
+  BGX_STATUS_NONFINITE = 2, /* q or qd not finite after the call */
+};
+
+typedef struct bgx_model bgx_model;
+
+/* Library version (major * 10000 + minor * 100 + patch). */
+int bgx_version(void);
+
+/* Message of the last error on this thread ("" if none). */
+const char* bgx_last_error(void);
+
+/* Topologies compiled into the library.  A topology is the static structure
+ * of a System (link types and parents, actuator and limit dof maps, contact
+ * candidates): the reference bakes the same fields into its compiled program
+ * (brax/base.py:471-480).  dims receives 8 ints:
+ * {num_links, q_size, qd_size, act_size, limit_rows, contact_candidates,
+ *  fast_contact_slots, model_table_floats}. */
+int bgx_num_topologies(void);
+const char* bgx_topology_name(int topo_id);
+const char* bgx_topology_signature(int topo_id);
+int bgx_topology_dims(int topo_id, int32_t* dims);
+
+/* Uploads a model table (float32, layout of brax_amd/codegen.py pack_model)
+ * for a topology.  Replaces brax.io.mjcf.load's output feeding the pipeline
+ * (brax/io/mjcf.py:317-489): the System arrays the step reads. */
+int bgx_model_create(int topo_id, const float* host_table, int64_t n_floats,
+                     bgx_model** out);
+int bgx_model_destroy(bgx_model* model);
+int bgx_model_topology(const bgx_model* model);
+
+/* pipeline.init (brax/generalized/pipeline.py:32-61): world link frames x
+ * and velocities xd of N envs from q (N, NQ) and qd (N, NV).  Outputs:
+ * x_pos (N, L, 3), x_rot (N, L, 4), xd_ang (N, L, 3), xd_vel (N, L, 3). */
+int bgx_init(const bgx_model* model, int32_t n_envs, const float* q,
+             const float* qd, float* x_pos, float* x_rot, float* xd_ang,
+             float* xd_vel, void* stream);
+
+/* pipeline.step (brax/generalized/pipeline.py:64-94) applied n_frames times
+ * with one action, as PipelineEnv.pipeline_step does (brax/envs/base.py:
+ * 128-137).  q (N, NQ) and qd (N, NV) are updated in place; act (N, NU) may
+ * be NULL (zero action).  x_* / xd_* (as in bgx_init) receive the frames at
+ * the new state when non-NULL; status (N) receives BGX_STATUS_* bits when
+ * non-NULL. */
+int bgx_step(const bgx_model* model, int32_t n_envs, int32_t n_frames,
+             float* q, float* qd, const float* act, float* x_pos,
+             float* x_rot, float* xd_ang, float* xd_vel, int32_t* status,
+             void* stream);
+
+/* n_steps env steps in one launch: step t applies act + t * act_step_stride
+ * (each (N, NU)) for n_frames substeps.  q_traj (n_steps, N, NQ) and qd_traj
+ * (n_steps, N, NV) receive the state after every env step when non-NULL.
+ * Other arguments as bgx_step.  The launch holds no host synchronization, so
+ * it can be captured in a HIP graph. */
+int bgx_rollout(const bgx_model* model, int32_t n_envs, int32_t n_steps,
+                int32_t n_frames, float* q, float* qd, const float* act,
+                int64_t act_step_stride, float* q_traj, float* qd_traj,
+                float* x_pos, float* x_rot, float* xd_ang, float* xd_vel,
+                int32_t* status, void* stream);
+
+/* Diagnostic variant of bgx_step: probe (N, 2 * NV + 3) receives, for the
+ * last substep, qf_smooth (dynamics.forward), qdd (integrator), and the
+ * active contact count, solver iterations and line-search trials. */
+int bgx_step_probe(const bgx_model* model, int32_t n_envs, int32_t n_frames,
+                   float* q, float* qd, const float* act, float* probe,
+                   void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BRAX_GEN_H_ */
