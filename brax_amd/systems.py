@@ -481,7 +481,16 @@ class ModelBuilder:
                             np.array(solimp, np.float64), solmix, name))
 
   def add_motor(self, joint: str, gear: float, ctrlrange=None):
-    self.motors.append((joint, float(gear), ctrlrange))
+    self.add_actuator(joint, gear, ctrlrange=ctrlrange)
+
+  def add_actuator(self, joint: str, gear: float, gain: float = 1.0,
+                   bias_q: float = 0.0, bias_qd: float = 0.0, ctrlrange=None,
+                   forcerange=None):
+    """Joint actuator: force = clip(gain*ctrl + gear*(bias_q*q +
+    bias_qd*qd), forcerange) * gear (MuJoCo motor: gain 1, no bias; position
+    servo with kp: gain kp, bias_q -kp)."""
+    self.motors.append((joint, float(gear), ctrlrange, float(gain),
+                        float(bias_q), float(bias_qd), forcerange))
 
   # -- compilation ----------------------------------------------------------
 
@@ -581,19 +590,23 @@ class ModelBuilder:
       for j in b.joints:
         q_start[j.name] = qi
         qi += 7 if j.typ == "free" else 1
-    q_id, qd_id, ctrl, gear = [], [], [], []
-    for jn, g, cr in self.motors:
+    q_id, qd_id, ctrl, gear, gain, bq, bqd, frc = [], [], [], [], [], [], [], []
+    for jn, g, cr, gn, b1, b2, fr in self.motors:
       q_id.append(q_start[jn])
       qd_id.append(joint_dof[jn])
       gear.append(g)
       ctrl.append(cr if cr is not None else (-np.inf, np.inf))
+      gain.append(gn)
+      bq.append(b1)
+      bqd.append(b2)
+      frc.append(fr if fr is not None else (-np.inf, np.inf))
     nu = len(q_id)
     actuator = Actuator(
         q_id=np.array(q_id, np.int64), qd_id=np.array(qd_id, np.int64),
         ctrl_range=np.array(ctrl, np.float64).reshape(nu, 2),
-        force_range=np.tile([-np.inf, np.inf], (nu, 1)).astype(np.float64),
-        gain=np.ones(nu), gear=np.array(gear, np.float64),
-        bias_q=np.zeros(nu), bias_qd=np.zeros(nu))
+        force_range=np.array(frc, np.float64).reshape(nu, 2),
+        gain=np.array(gain, np.float64), gear=np.array(gear, np.float64),
+        bias_q=np.array(bq, np.float64), bias_qd=np.array(bqd, np.float64))
 
     # qpos0: free joints at the body pose, hinges at zero
     init_q = []
@@ -884,7 +897,9 @@ def ant() -> System:
 
 
 def pendulum(n_links: int = 1, motor_gear: Optional[float] = None,
-             ctrlrange=(-100.0, 100.0), timestep: float = 0.001) -> System:
+             ctrlrange=(-100.0, 100.0), timestep: float = 0.001,
+             position_servo: Optional[Tuple[float, float, Tuple[float, float]]] = None
+             ) -> System:
   """Chain of n hinged unit-mass spheres (brax/test_data/*pendulum*.xml).
 
   Each body hangs 0.5 m along +y from its hinge (axis +x); spheres r=0.15,
@@ -903,6 +918,22 @@ def pendulum(n_links: int = 1, motor_gear: Optional[float] = None,
     parent = body
   if motor_gear is not None:
     b.add_motor("hinge1", motor_gear, ctrlrange)
+  if position_servo is not None:
+    kp, gear, frc = position_servo
+    b.add_actuator("hinge1", gear, gain=kp, bias_q=-kp, forcerange=frc)
+  return b.build()
+
+
+def free_body(gravity=(0.0, 0.0, -9.81), radius: float = 0.1,
+              mass: float = 1.0, timestep: float = 0.01,
+              pos=(0.0, 0.0, 1.0)) -> System:
+  """One free sphere with no contacts: closed-form integrator checks."""
+  b = ModelBuilder("free_body", timestep=timestep, iterations=4,
+                   gravity=gravity)
+  body = b.add_body("ball", 0, pos=pos)
+  b.add_free_joint(body)
+  b.add_geom(body, GEOM_SPHERE, (radius,), mass=mass, contype=0,
+             conaffinity=0, name="ball")
   return b.build()
 
 

@@ -1,0 +1,102 @@
+"""Writes the golden vectors of the GPU parity tests (run on the CPU).
+
+  python tests/golden/make_golden.py
+
+Every expected value comes from the float64 oracle (oracle/generalized.py),
+which the CPU tests pin against the reference's known answers and closed
+forms.  Files:
+  quadruped.npz  seeded synthetic-quadruped inputs (q noise U(-0.1, 0.1) on
+                 all coordinates, qd 0.1 N(0,1), act U(-1,1)); states after
+                 each of 20 env steps (5 substeps) with the exact inverse;
+                 the first-substep qf_smooth/qdd; the 1-step state with the
+                 reference's Newton-Schulz inverse and its reset flags; the
+                 1-step state of the float32 oracle
+  pendulum3.npz  contact-free triple pendulum, 20 env steps
+  ant.npz        ant, 1 env step (float64, and the float32 oracle)
+"""
+
+import os
+import sys
+
+import numpy as np
+import torch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from brax_amd import systems as S  # noqa: E402
+from oracle.generalized import Oracle  # noqa: E402
+
+F64 = torch.float64
+
+
+def _inputs(sys, n, steps, seed, noise=0.1, act_scale=1.0):
+  rng = np.random.default_rng(seed)
+  q0 = sys.init_q + rng.uniform(-noise, noise, (n, sys.q_size()))
+  qd0 = 0.1 * rng.standard_normal((n, sys.qd_size()))
+  acts = rng.uniform(-act_scale, act_scale, (steps, n, max(sys.act_size(), 0)))
+  return q0, qd0, acts
+
+
+def _trajectory(o, q0, qd0, acts, n_frames):
+  st = o.init(q0, qd0)
+  qs, qds = [], []
+  for a in acts:
+    st = o.env_step(st, a if a.shape[-1] else None, n_frames)
+    qs.append(st.q.numpy().copy())
+    qds.append(st.qd.numpy().copy())
+  return np.stack(qs), np.stack(qds)
+
+
+def quadruped(path):
+  sys = S.quadruped(0)
+  n, steps = 32, 20
+  q0, qd0, acts = _inputs(sys, n, steps, seed=0)
+  ex = Oracle(sys, F64, inverse="exact")
+  q_traj, qd_traj = _trajectory(ex, q0, qd0, acts, 5)
+  st1 = ex.step(ex.init(q0, qd0), acts[0])
+  o32 = Oracle(sys, torch.float32, inverse="exact")
+  st32 = o32.env_step(o32.init(q0, qd0), acts[0], 5)
+  ns = Oracle(sys, F64, inverse="ns")
+  st = ns.init(q0, qd0)
+  x_init = st.mass_mx_inv.clone()
+  reset = None
+  for f in range(5):
+    st = ns.step(st, acts[0])
+    if f == 0:
+      eye = torch.eye(sys.qd_size(), dtype=F64)
+      reset = (torch.linalg.norm((eye - st.mass_mx @ x_init).reshape(n, -1), dim=-1) > 1).numpy()
+  np.savez_compressed(
+      path, q0=q0, qd0=qd0, acts=acts, q_traj=q_traj, qd_traj=qd_traj,
+      qf_smooth1=st1.qf_smooth.numpy(), qdd1=st1.qdd.numpy(),
+      q1_ns=st.q.numpy(), qd1_ns=st.qd.numpy(), ns_reset=reset,
+      q1_f32=st32.q.double().numpy(), qd1_f32=st32.qd.double().numpy())
+
+
+def pendulum3(path):
+  sys = S.pendulum(3)
+  n, steps = 16, 20
+  rng = np.random.default_rng(1)
+  q0 = rng.uniform(-1, 1, (n, 3))
+  qd0 = rng.standard_normal((n, 3))
+  acts = np.zeros((steps, n, 0))
+  q_traj, qd_traj = _trajectory(Oracle(sys, F64, inverse="exact"), q0, qd0, acts, 5)
+  np.savez_compressed(path, q0=q0, qd0=qd0, q_traj=q_traj, qd_traj=qd_traj)
+
+
+def ant(path):
+  sys = S.ant()
+  n = 16
+  q0, qd0, acts = _inputs(sys, n, 1, seed=2)
+  q_traj, qd_traj = _trajectory(Oracle(sys, F64, inverse="exact"), q0, qd0, acts, 5)
+  o32 = Oracle(sys, torch.float32, inverse="exact")
+  st32 = o32.env_step(o32.init(q0, qd0), acts[0], 5)
+  np.savez_compressed(path, q0=q0, qd0=qd0, acts=acts, q_traj=q_traj, qd_traj=qd_traj,
+                      q1_f32=st32.q.double().numpy(), qd1_f32=st32.qd.double().numpy())
+
+
+if __name__ == "__main__":
+  quadruped(os.path.join(HERE, "quadruped.npz"))
+  pendulum3(os.path.join(HERE, "pendulum3.npz"))
+  ant(os.path.join(HERE, "ant.npz"))
+  print("written")
