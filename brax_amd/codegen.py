@@ -28,16 +28,17 @@ from brax_amd import systems as S
 # record layouts of the float32 model table (mirrored in csrc/bgx_layout.h)
 HEADER = 8  # gravity[3], dt, solver_iterations, solver_maxls, tol, ls_eps
 LINK_STRIDE = 24  # tpos[3] trot[4] jpos[3] ipos[3] irot[4] I[6] mass
-DOF_STRIDE = 20  # ang[3] vel[3] armature damping stiffness lo hi invweight solref[2] solimp[5]
+DOF_STRIDE = 24  # ang[3] vel[3] armature damping stiffness lo hi invweight imp[9]
 ACT_STRIDE = 8  # ctrl_lo ctrl_hi force_lo force_hi gain gear bias_q bias_qd
-CAND_STRIDE = 32  # gpos[3] gquat[4] radius halflen sign mu solref[2] solimp[5] tw ppos[3] pn[3] t1[3] t2[3]
+CAND_STRIDE = 36  # gpos[3] gquat[4] radius halflen sign mu imp[9] tw ppos[3] pn[3] t1[3] t2[3]
+IMP_SIZE = 9  # dmin dmax 1/width mid power 1/mid^(p-1) 1/(1-mid)^(p-1) b k
 
 KIND_SPHERE = 0
 KIND_CAPSULE = 1
 
 # fast-path contact-slot capacity per topology (other candidates take the
 # scratch path); sized from the active-contact distribution of the workload
-DEFAULT_SLOTS = 6
+DEFAULT_SLOTS = 5
 
 JAXOPT_TOL = 1e-3  # jaxopt ProximalGradient default tol
 LS_EPS = float(np.finfo(np.float32).eps)
@@ -54,6 +55,12 @@ class Topology:
     self.L = sys.num_links()
     self.nq, self.nv, self.nu = sys.q_size(), sys.qd_size(), sys.act_size()
     self.ltype = [0 if t == "f" else int(t) for t in sys.link_types]
+    for l, t in enumerate(sys.link_types):
+      if t == "f":  # the kernel hard-codes the free-joint motion (mjcf.py:354-356)
+        dr = sys.dof_ranges()[l]
+        if not (np.array_equal(sys.dof.motion.ang[dr], np.eye(6, 3, -3)) and
+                np.array_equal(sys.dof.motion.vel[dr], np.eye(6, 3))):
+          raise NotImplementedError("free joint with non-standard dof motion")
     self.qadr = [r[0] for r in sys.q_ranges()]
     self.dadr = [r[0] for r in sys.dof_ranges()]
     roots = []
@@ -104,6 +111,12 @@ class Topology:
         self.cand_kind.append(kind)
         self.cand_src.append((p, sign))
     self.ncand = len(self.cand_link)
+    # every contact plane is the z-up floor: frames and distances specialize
+    self.plane_z = all(
+        np.allclose(S.quat_to_mat(sys.geom_quat[p.geom1])[:, 2], [0, 0, 1], atol=0)
+        for p, _ in self.cand_src) if self.cand_src else True
+    # per dof: 1 for a pure rotation axis (motion.vel == 0)
+    self.dof_hinge = [int(not np.any(sys.dof.motion.vel[d])) for d in range(self.nv)]
     self.slots = max(1, min(slots, self.ncand)) if self.ncand else 1
     self.off_link = HEADER
     self.off_dof = self.off_link + LINK_STRIDE * self.L
@@ -113,7 +126,8 @@ class Topology:
 
   def signature(self) -> str:
     key = repr((self.link_types, self.parents, self.act_q, self.act_d,
-                self.lim_d, self.cand_link, self.cand_kind, self.slots))
+                self.lim_d, self.cand_link, self.cand_kind, self.slots,
+                self.plane_z, self.dof_hinge))
     return hashlib.sha1(key.encode()).hexdigest()[:16]
 
   def cpp_struct(self) -> str:
@@ -125,7 +139,7 @@ class Topology:
     s += f'  static constexpr const char* NAME = "{self.name}";\n'
     s += (f"  static constexpr int L = {self.L}, NQ = {self.nq}, NV = {self.nv}, "
           f"NU = {self.nu}, NLIM = {len(self.lim_d)}, NCAND = {self.ncand}, "
-          f"KC = {self.slots};\n")
+          f"KC = {self.slots}, PLANE_Z = {int(self.plane_z)};\n")
     s += (f"  static constexpr int OFF_LINK = {self.off_link}, OFF_DOF = "
           f"{self.off_dof}, OFF_ACT = {self.off_act}, OFF_CAND = "
           f"{self.off_cand}, SIZE = {self.size};\n")
@@ -137,9 +151,27 @@ class Topology:
     s += arr("ACT_Q", self.act_q) + arr("ACT_D", self.act_d)
     s += arr("LIM_D", self.lim_d) + arr("LIM_Q", self.lim_q)
     s += arr("CAND_LINK", self.cand_link) + arr("CAND_KIND", self.cand_kind)
+    s += arr("DOF_HINGE", self.dof_hinge)
     s += f'  static constexpr const char* SIG = "{self.signature()}";\n'
     s += "};\n"
     return s
+
+
+def impedance_record(solref, solimp) -> np.ndarray:
+  """Constants of constraint.py:29-65 for solver params solref(2)+solimp(5).
+
+  imp(pos) = clip(dmin + y (dmax - dmin)) with x = |pos| / width,
+  y = x^p / mid^(p-1) below mid, 1 - (1-x)^p / (1-mid)^(p-1) above;
+  aref = -b vel - k imp pos with b = 2 / (dmax timeconst) (or -dampratio /
+  dmax if dampratio <= 0) and k = 1 / (dmax^2 timeconst^2 dampratio^2) (or
+  -timeconst / dmax^2 if timeconst <= 0)."""
+  tc, dr = float(solref[0]), float(solref[1])
+  dmin, dmax, width, mid, power = [float(v) for v in solimp]
+  b = -dr / dmax if dr <= 0 else 2.0 / (dmax * tc)
+  k = -tc / (dmax * dmax) if tc <= 0 else 1.0 / (dmax * dmax * tc * tc * dr * dr)
+  return np.array([dmin, dmax, 1.0 / width, mid, power,
+                   1.0 / mid ** (power - 1), 1.0 / (1.0 - mid) ** (power - 1),
+                   b, k])
 
 
 def _ident(name: str) -> str:
@@ -180,7 +212,8 @@ def pack_model(sys: S.System, topo: Topology = None) -> np.ndarray:
     m[o + 9] = -np.inf if lo is None else lo[d]
     m[o + 10] = np.inf if hi is None else hi[d]
     m[o + 11] = sys.dof.invweight[d]
-    m[o + 12:o + 19] = sys.dof.solver_params[d]
+    sp = sys.dof.solver_params[d]
+    m[o + 12:o + 12 + IMP_SIZE] = impedance_record(sp[:2], sp[2:])
   a = sys.actuator
   for u in range(topo.nu):
     o = topo.off_act + u * ACT_STRIDE
@@ -197,22 +230,21 @@ def pack_model(sys: S.System, topo: Topology = None) -> np.ndarray:
     m[o + 8] = sys.geom_size[g2][1]
     m[o + 9] = sign
     m[o + 10] = p.friction[0]
-    m[o + 11:o + 13] = p.solref
-    m[o + 13:o + 18] = p.solimp
+    m[o + 11:o + 11 + IMP_SIZE] = impedance_record(p.solref, p.solimp)
     lb = int(sys.geom_bodyid[g2]) - 1
-    m[o + 18] = sys.link.invweight[lb]  # world side contributes 0
+    m[o + 20] = sys.link.invweight[lb]  # world side contributes 0
     # plane on the world body: pose = geom pose (contact.py:48-53)
     pr = S.quat_to_mat(sys.geom_quat[g1])
     n = pr[:, 2]
-    m[o + 19:o + 22] = sys.geom_pos[g1]
-    m[o + 22:o + 25] = n
+    m[o + 21:o + 24] = sys.geom_pos[g1]
+    m[o + 24:o + 27] = n
     # sphere frame: [n, b, n x b] with b = orthogonals(n) (math.py:247-254)
     nn = n / np.linalg.norm(n)
     b = np.array([0.0, 1.0, 0.0]) if -0.5 < nn[1] < 0.5 else np.array([0.0, 0.0, 1.0])
     b = b - nn * nn.dot(b)
     b = b / np.linalg.norm(b)
-    m[o + 25:o + 28] = b
-    m[o + 28:o + 31] = np.cross(nn, b)
+    m[o + 27:o + 30] = b
+    m[o + 30:o + 33] = np.cross(nn, b)
   return m.astype(np.float32)
 
 

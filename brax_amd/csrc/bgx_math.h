@@ -18,6 +18,13 @@ struct Q4 {
   float w, x, y, z;
 };
 
+// hardware reciprocal / square root / reciprocal square root (1 ulp); the
+// step's divisions are all normalizations or scale factors whose rounding is
+// far below the parity tolerance
+BGX_D float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+BGX_D float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+BGX_D float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+
 BGX_D V3 v3(float x, float y, float z) { return V3{x, y, z}; }
 BGX_D V3 v3zero() { return V3{0.f, 0.f, 0.f}; }
 BGX_D Q4 qident() { return Q4{1.f, 0.f, 0.f, 0.f}; }
@@ -72,23 +79,23 @@ BGX_D Q4 qnormalize(Q4 q) {
   const bool zero = fabsf(q.w) <= 1e-8f && fabsf(q.x) <= 1e-8f &&
                     fabsf(q.y) <= 1e-8f && fabsf(q.z) <= 1e-8f;
   const float n =
-      zero ? 0.f : sqrtf(q.w * q.w + q.x * q.x + q.y * q.y + q.z * q.z);
-  const float d = n + (n == 0.f ? 1e-6f : 0.f);
-  return Q4{q.w / d, q.x / d, q.y / d, q.z / d};
+      zero ? 0.f : fsqrt(q.w * q.w + q.x * q.x + q.y * q.y + q.z * q.z);
+  const float r = rcp(n + (n == 0.f ? 1e-6f : 0.f));
+  return Q4{q.w * r, q.x * r, q.y * r, q.z * r};
 }
 
 BGX_D V3 vnormalize(V3 v, float* norm_out) {
   const bool zero = fabsf(v.x) <= 1e-8f && fabsf(v.y) <= 1e-8f && fabsf(v.z) <= 1e-8f;
-  const float n = zero ? 0.f : sqrtf(dot(v, v));
-  const float d = n + (n == 0.f ? 1e-6f : 0.f);
+  const float n = zero ? 0.f : fsqrt(dot(v, v));
+  const float r = rcp(n + (n == 0.f ? 1e-6f : 0.f));
   if (norm_out) *norm_out = n;
-  return V3{v.x / d, v.y / d, v.z / d};
+  return V3{v.x * r, v.y * r, v.z * r};
 }
 
 // math.py:151-165 quat_to_3x3 (m[row][col])
 BGX_D void quat_to_3x3(Q4 q, float m[3][3]) {
   const float d = q.w * q.w + q.x * q.x + q.y * q.y + q.z * q.z;
-  const float s = 2.f / d;
+  const float s = 2.f * rcp(d);
   const float xs = q.x * s, ys = q.y * s, zs = q.z * s;
   const float wx = q.w * xs, wy = q.w * ys, wz = q.w * zs;
   const float xx = q.x * xs, xy = q.x * ys, xz = q.x * zs;

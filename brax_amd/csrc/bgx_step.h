@@ -60,6 +60,34 @@ BGX_D const float* cand_rec(const float* __restrict__ M, int c) {
   return M + T::OFF_CAND + c * CAND_STRIDE;
 }
 
+BGX_D V3 v3_(float x, float y, float z) { return V3{x, y, z}; }
+BGX_D V3 unit3(int k) {
+  return k == 0 ? v3(1.f, 0.f, 0.f) : (k == 1 ? v3(0.f, 1.f, 0.f) : v3(0.f, 0.f, 1.f));
+}
+
+// dof motion (brax/base.py DoF.motion): a free joint's six dofs are the
+// fixed unit translations then unit rotations (brax/io/mjcf.py:354-356), and
+// a hinge has no translational part, so both are compile-time constants
+template <class T>
+BGX_D V3 dof_ang(const float* __restrict__ M, int d) {
+  const int l = T::DOF_LINK[d];
+  if (T::LTYPE[l] == 0) {
+    const int k = d - T::DADR[l];
+    return k < 3 ? v3zero() : unit3(k - 3);
+  }
+  return ldv3(dof_rec<T>(M, d) + DF_ANG);
+}
+template <class T>
+BGX_D V3 dof_vel(const float* __restrict__ M, int d) {
+  const int l = T::DOF_LINK[d];
+  if (T::LTYPE[l] == 0) {
+    const int k = d - T::DADR[l];
+    return k < 3 ? unit3(k) : v3zero();
+  }
+  if (T::DOF_HINGE[d]) return v3zero();
+  return ldv3(dof_rec<T>(M, d) + DF_VEL);
+}
+
 template <class T>
 constexpr bool hosts_contact(int l) {
   for (int c = 0; c < T::NCAND; ++c)
@@ -90,8 +118,7 @@ BGX_D void forward_kinematics(const float* __restrict__ M, const float* q,
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         if (k < T::LTYPE[l]) {
-          const float* df = dof_rec<T>(M, da + k);
-          const V3 ang = ldv3(df + DF_ANG), vel = ldv3(df + DF_VEL);
+          const V3 ang = dof_ang<T>(M, da + k), vel = dof_vel<T>(M, da + k);
           const float qk = q[qa + k], qdk = qd[da + k];
           const Q4 rk = qnormalize(qrot_axis(ang, qk));
           const V3 pk = vel * qk, jda = ang * qdk, jdv = vel * qdk;
@@ -193,45 +220,58 @@ BGX_D void link_velocities(const Tree<T>& t, const float* z, V3* va, V3* vv) {
 }
 
 // generalized force of per-link spatial forces (fa: torque about the root
-// COM, fv: force), accumulated leaf to root (dynamics.py:174-182)
+// COM, fv: force), accumulated leaf to root (dynamics.py:174-182); each
+// link's dofs are projected as soon as its subtree sum is final
 template <class T>
-BGX_D void project_forces(const Tree<T>& t, V3* fa, V3* fv, float* tau) {
+BGX_D void project_forces(const Tree<T>& t, const V3* fa, const V3* fv, float* tau) {
+  V3 acc_a[T::L], acc_v[T::L];
+#pragma unroll
+  for (int l = 0; l < T::L; ++l) acc_a[l] = v3zero(), acc_v[l] = v3zero();
 #pragma unroll
   for (int l = T::L - 1; l >= 0; --l) {
+    const V3 a = fa[l] + acc_a[l], v = fv[l] + acc_v[l];
+    const int nd = T::LTYPE[l] == 0 ? 6 : T::LTYPE[l];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      if (k < nd) {
+        const int d = T::DADR[l] + k;
+        tau[d] = dot(t.cv[d], v) + dot(t.ca[d], a);
+      }
+    }
     const int par = T::PARENT[l];
     if (par >= 0) {
-      fa[par] = fa[par] + fa[l];
-      fv[par] = fv[par] + fv[l];
+      acc_a[par] = acc_a[par] + a;
+      acc_v[par] = acc_v[par] + v;
     }
-  }
-#pragma unroll
-  for (int d = 0; d < T::NV; ++d) {
-    const int l = T::DOF_LINK[d];
-    tau[d] = dot(t.cv[d], fv[l]) + dot(t.ca[d], fa[l]);
   }
 }
 
 // ---------------------------------------------------------------------------
-// impedance and reference acceleration (constraint.py:29-65)
+// impedance and reference acceleration (constraint.py:29-65); im is an
+// impedance record (bgx_layout.h) with the parameter-only factors
+// precomputed on the host
 
-BGX_D void imp_aref(const float* sp, float pos, float& imp, float& bcoef,
-                    float& kcoef) {
-  const float timeconst = sp[0], dampratio = sp[1], dmin = sp[2],
-              dmax = sp[3], width = sp[4], mid = sp[5], power = sp[6];
-  const float imp_x = fabsf(pos) / width;
-  const float imp_a = (1.f / powf(mid, power - 1.f)) * powf(imp_x, power);
-  const float imp_b =
-      1.f - (1.f / powf(1.f - mid, power - 1.f)) * powf(1.f - imp_x, power);
-  const float imp_y = imp_x < mid ? imp_a : imp_b;
-  float im = dmin + imp_y * (dmax - dmin);
-  im = clipf(im, dmin, dmax);
-  im = imp_x > 1.f ? dmax : im;
-  float b = 2.f / (dmax * timeconst);
-  float k = 1.f / (dmax * dmax * timeconst * timeconst * dampratio * dampratio);
-  b = dampratio <= 0.f ? -dampratio / dmax : b;
-  k = timeconst <= 0.f ? -timeconst / (dmax * dmax) : k;
-  imp = im, bcoef = b, kcoef = k;
+BGX_D float impedance(const float* __restrict__ im, float pos) {
+  const float dmin = im[IM_DMIN], dmax = im[IM_DMAX], mid = im[IM_MID],
+              power = im[IM_POWER];
+  const float x = fabsf(pos) * im[IM_INVWIDTH];
+  float xa, xb;
+  if (power == 2.f) {
+    xa = x * x;
+    xb = (1.f - x) * (1.f - x);
+  } else {
+    xa = powf(x, power);
+    xb = powf(1.f - x, power);
+  }
+  const float ia = im[IM_ACOEF] * xa;
+  const float ib = 1.f - im[IM_BCOEF] * xb;
+  float imp = dmin + (x < mid ? ia : ib) * (dmax - dmin);
+  imp = clipf(imp, dmin, dmax);
+  return x > 1.f ? dmax : imp;
 }
+
+// (1 - imp) / (imp + 1e-8): the diagonal regularizer factor
+BGX_D float imp_ratio(float imp) { return (1.f - imp) * rcp(imp + 1e-8f); }
 
 // ---------------------------------------------------------------------------
 // contact geometry of candidate c: MJX plane-sphere / plane-capsule
@@ -255,34 +295,92 @@ BGX_D void candidate_center(const float* __restrict__ M, int c, const V3* xp,
 template <class T>
 BGX_D float candidate_dist(const float* __restrict__ M, int c, V3 ctr) {
   const float* cd = cand_rec<T>(M, c);
+  if (T::PLANE_Z) return (ctr.z - cd[CD_PPOS + 2]) - cd[CD_RADIUS];
   return dot(ctr - ldv3(cd + CD_PPOS), ldv3(cd + CD_PN)) - cd[CD_RADIUS];
 }
 
 // ---------------------------------------------------------------------------
-// constraint solve (constraint.py:194-240 with jaxopt ProjectedGradient)
-// and integration (integrator.py:46-84) with KS contact slots
+// contact slots and projected-gradient row storage
 
-template <class T, int KS>
-BGX_D void solve_and_integrate(const float* __restrict__ M, float* q,
-                               float* qd, const Tree<T>& t, const V3* xp,
-                               const Q4* xr, const V3* cda, const V3* cdv,
-                               const float* qf_smooth, float* lds,
-                               float* dbg) {
+// geometry of one active contact: link, lever arm r = pos - root COM, and the
+// pyramid directions n + s mu t_k (constraint.py:160-164).  On the z-up floor
+// n = z and t1, t2 are horizontal, so mu t1 = (ax, ay, 0), mu t2 = (-ay, ax, 0)
+template <bool PZ>
+struct SlotGeo;
+template <>
+struct SlotGeo<true> {
+  int link;
+  V3 r;
+  float ax, ay;
+};
+template <>
+struct SlotGeo<false> {
+  int link;
+  V3 r, d0, d1, d2;
+};
+
+BGX_D void slot_set(SlotGeo<true>& g, int link, V3 r, V3 n, V3 d1, V3 d2) {
+  (void)n, (void)d2;
+  g.link = link, g.r = r, g.ax = d1.x, g.ay = d1.y;
+}
+BGX_D void slot_set(SlotGeo<false>& g, int link, V3 r, V3 n, V3 d1, V3 d2) {
+  g.link = link, g.r = r, g.d0 = n, g.d1 = d1, g.d2 = d2;
+}
+// J_c^T v for the 4 rows of a contact: the world force at the contact point
+BGX_D V3 slot_force(const SlotGeo<true>& g, float v0, float v1, float v2, float v3) {
+  const float a = v0 - v1, b = v2 - v3;
+  return v3_(g.ax * a - g.ay * b, g.ay * a + g.ax * b, (v0 + v1) + (v2 + v3));
+}
+BGX_D V3 slot_force(const SlotGeo<false>& g, float v0, float v1, float v2, float v3) {
+  return g.d0 * ((v0 + v1) + (v2 + v3)) + g.d1 * (v0 - v1) + g.d2 * (v2 - v3);
+}
+// (n.vp, mu t1.vp, mu t2.vp) for a contact-point velocity vp
+BGX_D void slot_dots(const SlotGeo<true>& g, V3 vp, float& e0, float& e1, float& e2) {
+  e0 = vp.z;
+  e1 = g.ax * vp.x + g.ay * vp.y;
+  e2 = g.ax * vp.y - g.ay * vp.x;
+}
+BGX_D void slot_dots(const SlotGeo<false>& g, V3 vp, float& e0, float& e1, float& e2) {
+  e0 = dot(g.d0, vp), e1 = dot(g.d1, vp), e2 = dot(g.d2, vp);
+}
+
+// per-row solver vectors of the slots: x, y (FISTA iterates), g (gradient),
+// r (residual), b.  Fast path: LDS, element e of this lane at lds[e * WAVE];
+// overflow path: a private array.
+constexpr int RV_X = 0, RV_Y = 1, RV_G = 2, RV_R = 3, RV_B = 4, RV_N = 5;
+template <int KS>
+struct LdsRows {
+  float* base;
+  BGX_D float& at(int v, int s, int k) const {
+    return base[((v * KS + s) * 4 + k) * WAVE];
+  }
+};
+template <int KS>
+struct PrivRows {
+  float a[RV_N][KS][4];
+  BGX_D float& at(int v, int s, int k) { return a[v][s][k]; }
+};
+
+// ---------------------------------------------------------------------------
+// one substep: pipeline.step (pipeline.py:64-94) with KS contact slots.
+// xp, xr: link frames at (q, qd) (kinematics.forward); lds: this lane's
+// link scratch (6 floats per link) followed by space for 54 more floats
+
+template <class T, int KS, class Rows>
+BGX_D void substep_impl(const float* __restrict__ M, float* q, float* qd,
+                        const float* act, const V3* xp, const Q4* xr, V3 rc0,
+                        float* lds, Rows& rows, float* dbg) {
   constexpr int NV = T::NV, L = T::L;
   constexpr int NL = T::NLIM > 0 ? T::NLIM : 1;
-
-  // unconstrained acceleration u = M^-1 qf_smooth and its link velocities
-  float u[NV];
+  constexpr bool PZ = T::PLANE_Z != 0;
+  Tree<T> t;
+  float* lds2 = lds + L * 6 * WAVE;  // second link-vector area (rows space)
 #pragma unroll
-  for (int d = 0; d < NV; ++d) u[d] = qf_smooth[d];
-  ltl_solve<T>(t, u);
-  V3 ua[L], uv[L];
-  link_velocities<T>(t, u, ua, uv);
+  for (int l = 0; l < L; ++l) t.rc[l] = rc0;
 
-  // ---- contact slots (constraint.py:135-176)
-  int sl_link[KS];
-  V3 sl_r[KS], sl_d0[KS], sl_d1[KS], sl_d2[KS];
-  float sl_diag[KS], sl_b[KS][4];
+  // ---- contacts (contact.py + MJX plane collisions) into slots
+  SlotGeo<PZ> sg[KS];
+  float sl_diag[KS], sl_bq[KS], sl_kt[KS];
   int n_c = 0;
 #pragma unroll
   for (int c = 0; c < T::NCAND; ++c) {
@@ -292,62 +390,296 @@ BGX_D void solve_and_integrate(const float* __restrict__ M, float* q,
     if (dist < 0.f) {
       const int l = T::CAND_LINK[c];
       const float* cd = cand_rec<T>(M, c);
-      const V3 n = ldv3(cd + CD_PN);
-      const float rad = cd[CD_RADIUS];
-      const V3 pos = ctr - n * (rad + 0.5f * dist);
+      const V3 n = PZ ? v3(0.f, 0.f, 1.f) : ldv3(cd + CD_PN);
+      const V3 pos = ctr - n * (cd[CD_RADIUS] + 0.5f * dist);
       V3 t1 = ldv3(cd + CD_T1), t2 = ldv3(cd + CD_T2);
       if (T::CAND_KIND[c] == KIND_CAPSULE) {
         // frame aligned with the capsule axis, fallback when near-normal
         float bn;
-        V3 b = vnormalize(axis - n * dot(n, axis), &bn);
+        const V3 b = vnormalize(axis - n * dot(n, axis), &bn);
         const V3 alt = (-0.5f < n.y && n.y < 0.5f) ? v3(0.f, 1.f, 0.f)
                                                     : v3(0.f, 0.f, 1.f);
         t1 = bn < 0.5f ? alt : b;
         t2 = cross(n, t1);
       }
       const float mu = cd[CD_MU];
-      const V3 r = pos - t.rc[l];
-      // row velocities J qd and J M^-1 qf_smooth from link velocities
-      const V3 vq = cdv[l] + cross(cda[l], r);
-      const V3 vu = uv[l] + cross(ua[l], r);
-      const V3 d1 = t1 * mu, d2 = t2 * mu;
-      float sp[7];
-#pragma unroll
-      for (int k = 0; k < 2; ++k) sp[k] = cd[CD_SOLREF + k];
-#pragma unroll
-      for (int k = 0; k < 5; ++k) sp[2 + k] = cd[CD_SOLIMP + k];
-      float imp, bc, kc;
-      imp_aref(sp, dist, imp, bc, kc);
+      SlotGeo<PZ> g;
+      slot_set(g, l, pos - t.rc[l], n, t1 * mu, t2 * mu);
+      const float* im = cd + CD_IMP;
+      const float imp = impedance(im, dist);
       const float tw = cd[CD_TW];
-      const float diag = (tw + mu * mu * tw) * (2.f * mu * mu * (1.f - imp) / (imp + 1e-8f));
-      const float e0q = dot(n, vq), e1q = dot(d1, vq), e2q = dot(d2, vq);
-      const float e0u = dot(n, vu), e1u = dot(d1, vu), e2u = dot(d2, vu);
-      const float velr[4] = {e0q + e1q, e0q - e1q, e0q + e2q, e0q - e2q};
-      const float jmu[4] = {e0u + e1u, e0u - e1u, e0u + e2u, e0u - e2u};
-      float bb[4];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const float aref = -bc * velr[k] - kc * imp * dist;
-        bb[k] = jmu[k] - aref;
-      }
+      const float diag = (tw + mu * mu * tw) * (2.f * mu * mu * imp_ratio(imp));
+      const float kt = im[IM_K] * imp * dist;
+      const float bq = im[IM_B];
       const int s = n_c;
 #pragma unroll
       for (int ss = 0; ss < KS; ++ss) {
         if (ss <= c && ss == s) {
-          sl_link[ss] = l;
-          sl_r[ss] = r;
-          sl_d0[ss] = n;
-          sl_d1[ss] = d1;
-          sl_d2[ss] = d2;
+          sg[ss] = g;
           sl_diag[ss] = diag;
-#pragma unroll
-          for (int k = 0; k < 4; ++k) sl_b[ss][k] = bb[k];
+          sl_bq[ss] = bq;
+          sl_kt[ss] = kt;
         }
       }
       n_c += 1;
     }
   }
-  if (n_c > KS) n_c = KS;  // callers choose KS >= active count
+  if (n_c > KS) n_c = KS;  // callers choose KS >= the active count
+
+  // ---- dynamics.transform_com (dynamics.py:27-89): cinr, cdof
+  V3 cc[L];
+  Sym3 ci[L];
+#pragma unroll
+  for (int l = 0; l < L; ++l) {
+    const float* lk = link_rec<T>(M, l);
+    const V3 xip = xp[l] + rotate(ldv3(lk + LK_IPOS), xr[l]);
+    const Q4 xir = qmul(xr[l], ldq4(lk + LK_IROT));
+    const float m = lk[LK_MASS];
+    const V3 p = xip - t.rc[l];
+    float R[3][3];
+    quat_to_3x3(xir, R);
+    const Sym3 I0 = ldsym(lk + LK_I);
+    float RI[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const V3 c = symmul(I0, v3(R[i][0], R[i][1], R[i][2]));
+      RI[i][0] = c.x, RI[i][1] = c.y, RI[i][2] = c.z;
+    }
+    auto rir = [&](int i, int j) {
+      return RI[i][0] * R[j][0] + RI[i][1] * R[j][1] + RI[i][2] * R[j][2];
+    };
+    const float pp = dot(p, p);
+    ci[l] = Sym3{rir(0, 0) + (pp - p.x * p.x) * m, rir(1, 1) + (pp - p.y * p.y) * m,
+                 rir(2, 2) + (pp - p.z * p.z) * m, rir(0, 1) - p.x * p.y * m,
+                 rir(0, 2) - p.x * p.z * m, rir(1, 2) - p.y * p.z * m};
+    cc[l] = p * m;
+  }
+#pragma unroll
+  for (int l = 0; l < L; ++l) {
+    const float* lk = link_rec<T>(M, l);
+    const int par = T::PARENT[l], da = T::DADR[l], qa = T::QADR[l];
+    V3 Pp = v3zero();
+    Q4 Pr = qident();
+    if (T::LTYPE[l] == 0) {
+      Pp = xp[l], Pr = xr[l];
+    } else if (par >= 0) {
+      Pp = xp[par], Pr = xr[par];
+    }
+    const V3 t1p = Pp + rotate(ldv3(lk + LK_TPOS), Pr);
+    const Q4 jr = qmul(Pr, ldq4(lk + LK_TROT));
+    const V3 jp = t1p + rotate(ldv3(lk + LK_JPOS), jr);
+    const int nd = T::LTYPE[l] == 0 ? 6 : T::LTYPE[l];
+    V3 accp = v3zero();
+    Q4 accr = qident();
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      if (k < nd) {
+        const int d = da + k;
+        const V3 ang = dof_ang<T>(M, d), vel = dof_vel<T>(M, d);
+        V3 a = ang, v = vel;
+        if (T::LTYPE[l] != 0) {  // stacked joints (dynamics.py:59-83)
+          a = rotate(ang, accr);
+          v = rotate(vel, accr) + cross(accp, a);
+          const Q4 rk = qnormalize(qrot_axis(ang, q[qa + k]));
+          accp = accp + rotate(vel * q[qa + k], accr);
+          accr = qmul(accr, rk);
+        }
+        a = rotate(a, jr);  // dynamics.py:86-89
+        t.ca[d] = a;
+        t.cv[d] = v - cross(t.rc[l] - jp, a);
+      }
+    }
+  }
+
+  // ---- cd: link velocities at qd (dynamics.py:90-103), also kept in lds2
+  V3 cda[L], cdv[L];
+  {
+    float qv[NV];
+#pragma unroll
+    for (int d = 0; d < NV; ++d) qv[d] = qd[d];
+    link_velocities<T>(t, qv, cda, cdv);
+  }
+#pragma unroll
+  for (int l = 0; l < L; ++l)
+    if (hosts_contact<T>(l)) {
+      float* f = lds2 + l * 6 * WAVE;
+      f[0] = cda[l].x, f[WAVE] = cda[l].y, f[2 * WAVE] = cda[l].z;
+      f[3 * WAVE] = cdv[l].x, f[4 * WAVE] = cdv[l].y, f[5 * WAVE] = cdv[l].z;
+    }
+
+  // ---- dynamics.inverse: RNEA bias (dynamics.py:105-184), cdofd on the fly
+  float qf[NV];
+  {
+    const V3 grav = ldv3(M + H_GRAVITY);
+    V3 aa[L], av[L], fa[L], fv[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      const int par = T::PARENT[l], da = T::DADR[l];
+      V3 a = par < 0 ? v3zero() : aa[par];
+      V3 v = par < 0 ? -grav : av[par];
+      if (T::LTYPE[l] == 0) {
+        // free joint: cdofd = (sum of the translational cdof qd) x cdof for
+        // the rotational dofs, zero for the translational ones (:107-112)
+        V3 sa = v3zero(), sv = v3zero();
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          sa = sa + t.ca[da + k] * qd[da + k];
+          sv = sv + t.cv[da + k] * qd[da + k];
+        }
+#pragma unroll
+        for (int k = 3; k < 6; ++k) {
+          V3 oa, ov;
+          cross_motion(sa, sv, t.ca[da + k], t.cv[da + k], oa, ov);
+          a = a + oa * qd[da + k];
+          v = v + ov * qd[da + k];
+        }
+      } else {
+        V3 sa = par >= 0 ? cda[par] : v3zero();
+        V3 sv = par >= 0 ? cdv[par] : v3zero();
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          if (k < T::LTYPE[l]) {
+            const int d = da + k;
+            if (k > 0) {
+              sa = sa + t.ca[d - 1] * qd[d - 1];
+              sv = sv + t.cv[d - 1] * qd[d - 1];
+            }
+            V3 oa, ov;
+            cross_motion(sa, sv, t.ca[d], t.cv[d], oa, ov);
+            a = a + oa * qd[d];
+            v = v + ov * qd[d];
+          }
+        }
+      }
+      aa[l] = a, av[l] = v;
+      const float m = link_rec<T>(M, l)[LK_MASS];
+      V3 f1a, f1v, f2a, f2v, f3a, f3v;
+      inertia_mul(cc[l], ci[l], m, a, v, f1a, f1v);
+      inertia_mul(cc[l], ci[l], m, cda[l], cdv[l], f2a, f2v);
+      cross_force(cda[l], cdv[l], f2a, f2v, f3a, f3v);
+      fa[l] = f1a + f3a, fv[l] = f1v + f3v;
+    }
+    project_forces<T>(t, fa, fv, qf);
+  }
+  // passive forces, actuators: qf_smooth = passive - bias + tau
+#pragma unroll
+  for (int d = 0; d < NV; ++d) {
+    const float* df = dof_rec<T>(M, d);
+    float p = -df[DF_DAMPING] * qd[d];
+    const int l = T::DOF_LINK[d];
+    if (T::LTYPE[l] != 0)
+      p = -q[T::QADR[l] + (d - T::DADR[l])] * df[DF_STIFFNESS] + p;
+    qf[d] = p - qf[d];
+  }
+#pragma unroll
+  for (int a = 0; a < T::NU; ++a) {
+    const float* ac = M + T::OFF_ACT + a * ACT_STRIDE;
+    const int qa = T::ACT_Q[a], da = T::ACT_D[a];
+    const float ctl = clipf(act[a], ac[AC_CTRL_LO], ac[AC_CTRL_HI]);
+    const float b = ac[AC_GEAR] * (q[qa] * ac[AC_BIAS_Q] + qd[da] * ac[AC_BIAS_QD]);
+    float f = ac[AC_GAIN] * ctl + b;
+    f = clipf(f, ac[AC_FORCE_LO], ac[AC_FORCE_HI]);
+    qf[da] += f * ac[AC_GEAR];
+  }
+  if (dbg) {
+#pragma unroll
+    for (int d = 0; d < NV; ++d) dbg[d] = qf[d];
+  }
+
+  // ---- mass.matrix: CRBA on the dof-ancestor pattern (mass.py:27-83), then
+  // the LTL factorization, leaves first
+  {
+    float km[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) km[l] = link_rec<T>(M, l)[LK_MASS];
+#pragma unroll
+    for (int l = L - 1; l >= 0; --l) {
+      const int par = T::PARENT[l];
+      if (par >= 0) {
+        cc[par] = cc[par] + cc[l];
+        ci[par] = symadd(ci[par], ci[l]);
+        km[par] = km[par] + km[l];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      const int l = T::DOF_LINK[i];
+      V3 fa, fv;
+      inertia_mul(cc[l], ci[l], km[l], t.ca[i], t.cv[i], fa, fv);
+      t.Lm[i][i] = dot(t.cv[i], fv) + dot(t.ca[i], fa) + dof_rec<T>(M, i)[DF_ARMATURE];
+#pragma unroll
+      for (int j = 0; j < i; ++j)
+        if ((T::DANC[i] >> j) & 1u) t.Lm[i][j] = dot(t.cv[j], fv) + dot(t.ca[j], fa);
+    }
+  }
+#pragma unroll
+  for (int k = NV - 1; k >= 0; --k) {
+    const float lkk = fsqrt(t.Lm[k][k]);
+    t.Lm[k][k] = lkk;
+    t.inv[k] = rcp(lkk);
+#pragma unroll
+    for (int j = 0; j < k; ++j)
+      if ((T::DANC[k] >> j) & 1u) t.Lm[k][j] *= t.inv[k];
+#pragma unroll
+    for (int i = 0; i < k; ++i) {
+      if ((T::DANC[k] >> i) & 1u) {
+        const float a = t.Lm[k][i];
+        t.Lm[i][i] -= a * a;
+#pragma unroll
+        for (int j = 0; j < i; ++j)
+          if ((T::DANC[i] >> j) & 1u) t.Lm[i][j] -= a * t.Lm[k][j];
+      }
+    }
+  }
+
+  // ---- unconstrained acceleration u = M^-1 qf_smooth
+  float u[NV];
+#pragma unroll
+  for (int d = 0; d < NV; ++d) u[d] = qf[d];
+  ltl_solve<T>(t, u);
+
+  // ---- contact rows: b = J u - aref, aref = -b J qd - k imp dist
+  // (constraint.py:58-65, :205-210); link velocities from lds (u) and lds2 (qd)
+  {
+    V3 ua[L], uv[L];
+    link_velocities<T>(t, u, ua, uv);
+#pragma unroll
+    for (int l = 0; l < L; ++l)
+      if (hosts_contact<T>(l)) {
+        float* f = lds + l * 6 * WAVE;
+        f[0] = ua[l].x, f[WAVE] = ua[l].y, f[2 * WAVE] = ua[l].z;
+        f[3 * WAVE] = uv[l].x, f[4 * WAVE] = uv[l].y, f[5 * WAVE] = uv[l].z;
+      }
+  }
+  float sb[KS][4];
+#pragma unroll
+  for (int s = 0; s < KS; ++s) {
+    if (s < n_c) {
+      const float* fu = lds + sg[s].link * 6 * WAVE;
+      const float* fq = lds2 + sg[s].link * 6 * WAVE;
+      const V3 r = sg[s].r;
+      const V3 vu = v3(fu[3 * WAVE], fu[4 * WAVE], fu[5 * WAVE]) +
+                    cross(v3(fu[0], fu[WAVE], fu[2 * WAVE]), r);
+      const V3 vq = v3(fq[3 * WAVE], fq[4 * WAVE], fq[5 * WAVE]) +
+                    cross(v3(fq[0], fq[WAVE], fq[2 * WAVE]), r);
+      float e0u, e1u, e2u, e0q, e1q, e2q;
+      slot_dots(sg[s], vu, e0u, e1u, e2u);
+      slot_dots(sg[s], vq, e0q, e1q, e2q);
+      const float bq = sl_bq[s], kt = sl_kt[s];
+      sb[s][0] = (e0u + e1u) + (bq * (e0q + e1q) + kt);
+      sb[s][1] = (e0u - e1u) + (bq * (e0q - e1q) + kt);
+      sb[s][2] = (e0u + e2u) + (bq * (e0q + e2q) + kt);
+      sb[s][3] = (e0u - e2u) + (bq * (e0q - e2q) + kt);
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < KS; ++s)
+    if (s < n_c)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        rows.at(RV_B, s, k) = sb[s][k];
+        rows.at(RV_X, s, k) = rows.at(RV_Y, s, k) = 0.f;
+      }
 
   // ---- limit rows (constraint.py:101-132), one per non-free dof
   float lside[NL], ldiag[NL], lb[NL];
@@ -361,23 +693,17 @@ BGX_D void solve_and_integrate(const float* __restrict__ M, float* q,
     const float pos = fminf(fminf(pmin, pmax), 0.f);
     const bool act = pos < 0.f;
     const float side = act ? (pmin < pmax ? 1.f : -1.f) : 0.f;
-    float imp, bc, kc;
-    float sp[7];
-#pragma unroll
-    for (int k = 0; k < 2; ++k) sp[k] = df[DF_SOLREF + k];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) sp[2 + k] = df[DF_SOLIMP + k];
-    imp_aref(sp, pos, imp, bc, kc);
+    const float* im = df + DF_IMP;
+    const float imp = impedance(im, pos);
     const float vel = side * qd[d];
-    const float aref = act ? (-bc * vel - kc * imp * pos) : 0.f;
+    const float aref = act ? (-im[IM_B] * vel - im[IM_K] * imp * pos) : 0.f;
     lside[i] = side;
-    ldiag[i] = act ? df[DF_INVWEIGHT] * (1.f - imp) / (imp + 1e-8f) : 0.f;
+    ldiag[i] = act ? df[DF_INVWEIGHT] * imp_ratio(imp) : 0.f;
     lb[i] = side * u[d] - aref;
     any_lim = any_lim || act;
   }
 
   // ---- products with A = J M^-1 J^T + diag, evaluated through the tree
-  // link scratch in LDS: element e of this lane at lds[e * WAVE]
   auto apply_jt = [&](auto in_slot, auto in_lim, float* tau) {
 #pragma unroll
     for (int l = 0; l < L; ++l)
@@ -388,12 +714,10 @@ BGX_D void solve_and_integrate(const float* __restrict__ M, float* q,
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
       if (s < n_c) {
-        const float v0 = in_slot(s, 0), v1 = in_slot(s, 1);
-        const float v2 = in_slot(s, 2), v3_ = in_slot(s, 3);
-        const V3 w = sl_d0[s] * (v0 + v1 + v2 + v3_) + sl_d1[s] * (v0 - v1) +
-                     sl_d2[s] * (v2 - v3_);
-        const V3 ta = cross(sl_r[s], w);
-        float* f = lds + sl_link[s] * 6 * WAVE;
+        const V3 w = slot_force(sg[s], in_slot(s, 0), in_slot(s, 1),
+                                in_slot(s, 2), in_slot(s, 3));
+        const V3 ta = cross(sg[s].r, w);
+        float* f = lds + sg[s].link * 6 * WAVE;
         f[0 * WAVE] += ta.x;
         f[1 * WAVE] += ta.y;
         f[2 * WAVE] += ta.z;
@@ -422,24 +746,25 @@ BGX_D void solve_and_integrate(const float* __restrict__ M, float* q,
     float z[NV];
     apply_jt(in_slot, in_lim, z);
     ltl_solve<T>(t, z);
-    V3 va[L], vv[L];
-    link_velocities<T>(t, z, va, vv);
+    {
+      V3 va[L], vv[L];
+      link_velocities<T>(t, z, va, vv);
 #pragma unroll
-    for (int l = 0; l < L; ++l)
-      if (hosts_contact<T>(l)) {
-        float* f = lds + l * 6 * WAVE;
-        f[0] = va[l].x, f[WAVE] = va[l].y, f[2 * WAVE] = va[l].z;
-        f[3 * WAVE] = vv[l].x, f[4 * WAVE] = vv[l].y, f[5 * WAVE] = vv[l].z;
-      }
+      for (int l = 0; l < L; ++l)
+        if (hosts_contact<T>(l)) {
+          float* f = lds + l * 6 * WAVE;
+          f[0] = va[l].x, f[WAVE] = va[l].y, f[2 * WAVE] = va[l].z;
+          f[3 * WAVE] = vv[l].x, f[4 * WAVE] = vv[l].y, f[5 * WAVE] = vv[l].z;
+        }
+    }
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
       if (s < n_c) {
-        const float* f = lds + sl_link[s] * 6 * WAVE;
+        const float* f = lds + sg[s].link * 6 * WAVE;
         const V3 a = v3(f[0], f[WAVE], f[2 * WAVE]);
         const V3 v = v3(f[3 * WAVE], f[4 * WAVE], f[5 * WAVE]);
-        const V3 vp = v + cross(a, sl_r[s]);
-        const float e0 = dot(sl_d0[s], vp), e1 = dot(sl_d1[s], vp),
-                    e2 = dot(sl_d2[s], vp);
+        float e0, e1, e2;
+        slot_dots(sg[s], v + cross(a, sg[s].r), e0, e1, e2);
         const float dg = sl_diag[s];
         out_slot(s, 0, e0 + e1 + dg * in_slot(s, 0));
         out_slot(s, 1, e0 - e1 + dg * in_slot(s, 1));
@@ -453,12 +778,7 @@ BGX_D void solve_and_integrate(const float* __restrict__ M, float* q,
   };
 
   // ---- jaxopt.ProjectedGradient: FISTA + backtracking line search from 0
-  float sx[KS][4], sy[KS][4], sg[KS][4], sr[KS][4];
   float lx[NL], ly[NL], lg[NL], lr[NL];
-#pragma unroll
-  for (int s = 0; s < KS; ++s)
-#pragma unroll
-    for (int k = 0; k < 4; ++k) sx[s][k] = sy[s][k] = sg[s][k] = sr[s][k] = 0.f;
 #pragma unroll
   for (int i = 0; i < NL; ++i) lx[i] = ly[i] = lg[i] = lr[i] = 0.f;
 
@@ -473,11 +793,11 @@ BGX_D void solve_and_integrate(const float* __restrict__ M, float* q,
     if (!running) break;
     // gradient at y: r = A y + b, g = A^T r
     float fy = 0.f;
-    apply_a([&](int s, int k) { return sy[s][k]; },
+    apply_a([&](int s, int k) { return rows.at(RV_Y, s, k); },
             [&](int i) { return ly[i]; },
             [&](int s, int k, float o) {
-              const float r = o + sl_b[s][k];
-              sr[s][k] = r;
+              const float r = o + rows.at(RV_B, s, k);
+              rows.at(RV_R, s, k) = r;
               fy += 0.5f * r * r;
             },
             [&](int i, float o) {
@@ -485,33 +805,34 @@ BGX_D void solve_and_integrate(const float* __restrict__ M, float* q,
               lr[i] = r;
               fy += 0.5f * r * r;
             });
-    apply_a([&](int s, int k) { return sr[s][k]; },
+    apply_a([&](int s, int k) { return rows.at(RV_R, s, k); },
             [&](int i) { return lr[i]; },
-            [&](int s, int k, float o) { sg[s][k] = o; },
+            [&](int s, int k, float o) { rows.at(RV_G, s, k) = o; },
             [&](int i, float o) { lg[i] = o; });
-    // line search: trial x+ = relu(y - eta g), evaluated into sr/lr
+    // line search: trial x+ = relu(y - eta g), residual into r
     bool cond = true;
     for (int ls = 0; ls <= maxls && cond; ++ls) {
       if (ls > 0) eta *= 0.5f;
       const float e_ = eta;
       float fx = 0.f, dg = 0.f, dd = 0.f;
-      apply_a([&](int s, int k) { return fmaxf(sy[s][k] - e_ * sg[s][k], 0.f); },
+      apply_a([&](int s, int k) {
+                return fmaxf(rows.at(RV_Y, s, k) - e_ * rows.at(RV_G, s, k), 0.f);
+              },
               [&](int i) { return fmaxf(ly[i] - e_ * lg[i], 0.f); },
               [&](int s, int k, float o) {
-                const float r = o + sl_b[s][k];
-                sr[s][k] = r;
+                const float r = o + rows.at(RV_B, s, k);
+                rows.at(RV_R, s, k) = r;
                 fx += 0.5f * r * r;
-                const float xp = fmaxf(sy[s][k] - e_ * sg[s][k], 0.f);
-                const float df = xp - sy[s][k];
-                dg += df * sg[s][k];
+                const float yv = rows.at(RV_Y, s, k), gv = rows.at(RV_G, s, k);
+                const float df = fmaxf(yv - e_ * gv, 0.f) - yv;
+                dg += df * gv;
                 dd += df * df;
               },
               [&](int i, float o) {
                 const float r = o + lb[i];
                 lr[i] = r;
                 fx += 0.5f * r * r;
-                const float xp = fmaxf(ly[i] - e_ * lg[i], 0.f);
-                const float df = xp - ly[i];
+                const float df = fmaxf(ly[i] - e_ * lg[i], 0.f) - ly[i];
                 dg += df * lg[i];
                 dd += df * df;
               });
@@ -521,10 +842,10 @@ BGX_D void solve_and_integrate(const float* __restrict__ M, float* q,
     // error at x+: |relu(x+ - grad f(x+)) - x+| with grad = A^T (A x+ + b)
     float e2 = 0.f;
     const float e_ = eta;
-    apply_a([&](int s, int k) { return sr[s][k]; },
+    apply_a([&](int s, int k) { return rows.at(RV_R, s, k); },
             [&](int i) { return lr[i]; },
             [&](int s, int k, float o) {
-              const float xp = fmaxf(sy[s][k] - e_ * sg[s][k], 0.f);
+              const float xp = fmaxf(rows.at(RV_Y, s, k) - e_ * rows.at(RV_G, s, k), 0.f);
               const float dv = fmaxf(xp - o, 0.f) - xp;
               e2 += dv * dv;
             },
@@ -533,17 +854,17 @@ BGX_D void solve_and_integrate(const float* __restrict__ M, float* q,
               const float dv = fmaxf(xp - o, 0.f) - xp;
               e2 += dv * dv;
             });
-    err = sqrtf(e2);
-    const float tn = 0.5f * (1.f + sqrtf(1.f + 4.f * tk * tk));
-    const float beta = (tk - 1.f) / tn;
+    err = fsqrt(e2);
+    const float tn = 0.5f * (1.f + fsqrt(1.f + 4.f * tk * tk));
+    const float beta = (tk - 1.f) * rcp(tn);
 #pragma unroll
     for (int s = 0; s < KS; ++s)
       if (s < n_c)
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-          const float xp = fmaxf(sy[s][k] - e_ * sg[s][k], 0.f);
-          sy[s][k] = xp + beta * (xp - sx[s][k]);
-          sx[s][k] = xp;
+          const float xp = fmaxf(rows.at(RV_Y, s, k) - e_ * rows.at(RV_G, s, k), 0.f);
+          rows.at(RV_Y, s, k) = xp + beta * (xp - rows.at(RV_X, s, k));
+          rows.at(RV_X, s, k) = xp;
         }
 #pragma unroll
     for (int i = 0; i < T::NLIM; ++i) {
@@ -551,7 +872,7 @@ BGX_D void solve_and_integrate(const float* __restrict__ M, float* q,
       ly[i] = xp + beta * (xp - lx[i]);
       lx[i] = xp;
     }
-    eta = eta <= 1e-6f ? 1.f : eta / 0.5f;
+    eta = eta <= 1e-6f ? 1.f : eta * 2.f;
     tk = tn;
     iters += 1;
   }
@@ -559,7 +880,7 @@ BGX_D void solve_and_integrate(const float* __restrict__ M, float* q,
   // ---- qdd = M^-1 (qf_smooth + J^T x) (integrator.py:58-64)
   float qdd[NV];
   if (has_rows) {
-    apply_jt([&](int s, int k) { return sx[s][k]; },
+    apply_jt([&](int s, int k) { return rows.at(RV_X, s, k); },
              [&](int i) { return lx[i]; }, qdd);
     ltl_solve<T>(t, qdd);
 #pragma unroll
@@ -568,15 +889,12 @@ BGX_D void solve_and_integrate(const float* __restrict__ M, float* q,
 #pragma unroll
     for (int d = 0; d < NV; ++d) qdd[d] = u[d];
   }
-  if (dbg) {  // per-env probe of the last substep: qf_smooth, qdd, counts
-    int o = 0;
+  if (dbg) {  // per-env probe of the last substep: qdd, counts
 #pragma unroll
-    for (int d = 0; d < NV; ++d) dbg[o++] = qf_smooth[d];
-#pragma unroll
-    for (int d = 0; d < NV; ++d) dbg[o++] = qdd[d];
-    dbg[o++] = (float)n_c;
-    dbg[o++] = (float)iters;
-    dbg[o++] = (float)trials;
+    for (int d = 0; d < NV; ++d) dbg[NV + d] = qdd[d];
+    dbg[2 * NV] = (float)n_c;
+    dbg[2 * NV + 1] = (float)iters;
+    dbg[2 * NV + 2] = (float)trials;
   }
 
   // ---- semi-implicit Euler (integrator.py:26-84)
@@ -588,14 +906,15 @@ BGX_D void solve_and_integrate(const float* __restrict__ M, float* q,
     const int qa = T::QADR[l], da = T::DADR[l];
     if (T::LTYPE[l] == 0) {
       const V3 ang = v3(qd[da + 3], qd[da + 4], qd[da + 5]);
-      const float an = sqrtf(dot(ang, ang)) + 1e-8f;
-      const Q4 qr = qrot_axis(v3(ang.x / an, ang.y / an, ang.z / an), dt * an);
+      const float an = fsqrt(dot(ang, ang)) + 1e-8f;
+      const float ian = rcp(an);
+      const Q4 qr = qrot_axis(v3(ang.x * ian, ang.y * ian, ang.z * ian), dt * an);
       Q4 rot = qmul(Q4{q[qa + 3], q[qa + 4], q[qa + 5], q[qa + 6]}, qr);
-      const float rn = sqrtf(rot.w * rot.w + rot.x * rot.x + rot.y * rot.y + rot.z * rot.z);
-      q[qa + 3] = rot.w / rn;
-      q[qa + 4] = rot.x / rn;
-      q[qa + 5] = rot.y / rn;
-      q[qa + 6] = rot.z / rn;
+      const float rn = rsq(rot.w * rot.w + rot.x * rot.x + rot.y * rot.y + rot.z * rot.z);
+      q[qa + 3] = rot.w * rn;
+      q[qa + 4] = rot.x * rn;
+      q[qa + 5] = rot.y * rn;
+      q[qa + 6] = rot.z * rn;
 #pragma unroll
       for (int k = 0; k < 3; ++k) q[qa + k] = q[qa + k] + qd[da + k] * dt;
     } else {
@@ -606,275 +925,32 @@ BGX_D void solve_and_integrate(const float* __restrict__ M, float* q,
   }
 }
 
-// ---------------------------------------------------------------------------
-// one substep: pipeline.step (pipeline.py:64-94)
-
+// one substep: kinematics at (q, qd), then the fast path (active contacts in
+// KC register/LDS slots) or, when more candidates touch, the overflow path
+// (all candidates, private arrays)
 template <class T>
 BGX_D int substep(const float* __restrict__ M, float* q, float* qd,
                   const float* act, float* lds, float* dbg) {
-  constexpr int NV = T::NV, L = T::L;
-  Tree<T> t;
-
-  // kinematics at (q, qd)
-  V3 xp[L];
-  Q4 xr[L];
+  static_assert(T::L > 0, "empty topology");
+  V3 xp[T::L];
+  Q4 xr[T::L];
   forward_kinematics<T, false>(M, q, qd, xp, xr, nullptr, nullptr);
-
-  // dynamics.transform_com (dynamics.py:27-134)
-  V3 xip[L];
-  Q4 xir[L];
-#pragma unroll
-  for (int l = 0; l < L; ++l) {
-    const float* lk = link_rec<T>(M, l);
-    xip[l] = xp[l] + rotate(ldv3(lk + LK_IPOS), xr[l]);
-    xir[l] = qmul(xr[l], ldq4(lk + LK_IROT));
-  }
-#pragma unroll
-  for (int l = 0; l < L; ++l) {
-    if (T::ROOT[l] == l) {
-      V3 s = v3zero();
-      float ms = 0.f;
-#pragma unroll
-      for (int l2 = 0; l2 < L; ++l2)
-        if (T::ROOT[l2] == l) {
-          const float m = link_rec<T>(M, l2)[LK_MASS];
-          s = s + xip[l2] * m;
-          ms = ms + m;
-        }
-      t.rc[l] = v3(s.x / ms, s.y / ms, s.z / ms);
-    }
-  }
-#pragma unroll
-  for (int l = 0; l < L; ++l) t.rc[l] = t.rc[T::ROOT[l]];
-  V3 cc[L];
-  Sym3 ci[L];
-#pragma unroll
-  for (int l = 0; l < L; ++l) {
-    const float* lk = link_rec<T>(M, l);
-    const float m = lk[LK_MASS];
-    const V3 p = xip[l] - t.rc[l];
-    float R[3][3];
-    quat_to_3x3(xir[l], R);
-    const Sym3 I0 = ldsym(lk + LK_I);
-    // R I R^T
-    float RI[3][3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const V3 row = v3(R[a][0], R[a][1], R[a][2]);
-      const V3 c = symmul(I0, row);
-      RI[a][0] = c.x, RI[a][1] = c.y, RI[a][2] = c.z;
-    }
-    auto rir = [&](int a, int b) {
-      return RI[a][0] * R[b][0] + RI[a][1] * R[b][1] + RI[a][2] * R[b][2];
-    };
-    const float pp = dot(p, p);
-    ci[l] = Sym3{rir(0, 0) + (pp - p.x * p.x) * m, rir(1, 1) + (pp - p.y * p.y) * m,
-                 rir(2, 2) + (pp - p.z * p.z) * m, rir(0, 1) - p.x * p.y * m,
-                 rir(0, 2) - p.x * p.z * m, rir(1, 2) - p.y * p.z * m};
-    cc[l] = p * m;
-  }
-
-  // cdof: dof motions in the root-COM frame (dynamics.py:47-89)
-#pragma unroll
-  for (int l = 0; l < L; ++l) {
-    const float* lk = link_rec<T>(M, l);
-    const int par = T::PARENT[l], da = T::DADR[l], qa = T::QADR[l];
-    V3 Pp = v3zero();
-    Q4 Pr = qident();
-    if (T::LTYPE[l] == 0) {
-      Pp = xp[l], Pr = xr[l];
-    } else if (par >= 0) {
-      Pp = xp[par], Pr = xr[par];
-    }
-    const V3 t1p = Pp + rotate(ldv3(lk + LK_TPOS), Pr);
-    const Q4 jr = qmul(Pr, ldq4(lk + LK_TROT));
-    const V3 jp = t1p + rotate(ldv3(lk + LK_JPOS), jr);
-    const int nd = T::LTYPE[l] == 0 ? 6 : T::LTYPE[l];
-    V3 accp = v3zero();
-    Q4 accr = qident();
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      if (k < nd) {
-        const int d = da + k;
-        const float* df = dof_rec<T>(M, d);
-        const V3 ang = ldv3(df + DF_ANG), vel = ldv3(df + DF_VEL);
-        V3 a = ang, v = vel;
-        if (T::LTYPE[l] != 0) {
-          a = rotate(ang, accr);
-          v = rotate(vel, accr) + cross(accp, a);
-          const Q4 rk = qnormalize(qrot_axis(ang, q[qa + k]));
-          accp = accp + rotate(vel * q[qa + k], accr);
-          accr = qmul(accr, rk);
-        }
-        a = rotate(a, jr);
-        const V3 off = t.rc[l] - jp;
-        t.ca[d] = a;
-        t.cv[d] = v - cross(off, a);
-      }
-    }
-  }
-
-  // cd: link velocities in the root-COM frame (dynamics.py:90-103)
-  V3 cda[L], cdv[L];
+  // root COM of the tree (dynamics.py:37-43); one root per topology
+  static_assert(T::ROOT[T::L - 1] == 0, "one kinematic tree per topology");
+  V3 rc;
   {
-    float qv[NV];
+    V3 sm = v3zero();
+    float ms = 0.f;
 #pragma unroll
-    for (int d = 0; d < NV; ++d) qv[d] = qd[d];
-    link_velocities<T>(t, qv, cda, cdv);
-  }
-
-  // cdofd (dynamics.py:105-130)
-  V3 dda[NV], ddv[NV];
-#pragma unroll
-  for (int l = 0; l < L; ++l) {
-    const int par = T::PARENT[l], da = T::DADR[l];
-    if (T::LTYPE[l] == 0) {
-      V3 sa = v3zero(), sv = v3zero();
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        sa = sa + t.ca[da + k] * qd[da + k];
-        sv = sv + t.cv[da + k] * qd[da + k];
-      }
-#pragma unroll
-      for (int k = 0; k < 6; ++k) {
-        if (k < 3) {
-          dda[da + k] = v3zero(), ddv[da + k] = v3zero();
-        } else {
-          cross_motion(sa, sv, t.ca[da + k], t.cv[da + k], dda[da + k], ddv[da + k]);
-        }
-      }
-    } else {
-      V3 sa = par >= 0 ? cda[par] : v3zero();
-      V3 sv = par >= 0 ? cdv[par] : v3zero();
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        if (k < T::LTYPE[l]) {
-          const int d = da + k;
-          if (k > 0) {
-            sa = sa + t.ca[d - 1] * qd[d - 1];
-            sv = sv + t.cv[d - 1] * qd[d - 1];
-          }
-          cross_motion(sa, sv, t.ca[d], t.cv[d], dda[d], ddv[d]);
-        }
-      }
+    for (int l = 0; l < T::L; ++l) {
+      const float* lk = link_rec<T>(M, l);
+      const float m = lk[LK_MASS];
+      sm = sm + (xp[l] + rotate(ldv3(lk + LK_IPOS), xr[l])) * m;
+      ms = ms + m;
     }
+    const float ims = rcp(ms);
+    rc = v3(sm.x * ims, sm.y * ims, sm.z * ims);
   }
-
-  // dynamics.inverse: RNEA bias (dynamics.py:137-184)
-  float bias[NV];
-  {
-    const V3 grav = ldv3(M + H_GRAVITY);
-    V3 aa[L], av[L], fa[L], fv[L];
-#pragma unroll
-    for (int l = 0; l < L; ++l) {
-      const int par = T::PARENT[l];
-      V3 a = par < 0 ? v3zero() : aa[par];
-      V3 v = par < 0 ? -grav : av[par];
-      const int nd = T::LTYPE[l] == 0 ? 6 : T::LTYPE[l];
-#pragma unroll
-      for (int k = 0; k < 6; ++k) {
-        if (k < nd) {
-          const int d = T::DADR[l] + k;
-          a = a + dda[d] * qd[d];
-          v = v + ddv[d] * qd[d];
-        }
-      }
-      aa[l] = a, av[l] = v;
-      const float m = link_rec<T>(M, l)[LK_MASS];
-      V3 f1a, f1v, f2a, f2v, f3a, f3v;
-      inertia_mul(cc[l], ci[l], m, a, v, f1a, f1v);
-      inertia_mul(cc[l], ci[l], m, cda[l], cdv[l], f2a, f2v);
-      cross_force(cda[l], cdv[l], f2a, f2v, f3a, f3v);
-      fa[l] = f1a + f3a, fv[l] = f1v + f3v;
-    }
-    project_forces<T>(t, fa, fv, bias);
-  }
-
-  // passive forces, actuators, qf_smooth (dynamics.py:187-236, actuator.py)
-  float qf[NV];
-#pragma unroll
-  for (int d = 0; d < NV; ++d) {
-    const float* df = dof_rec<T>(M, d);
-    qf[d] = -df[DF_DAMPING] * qd[d];
-  }
-#pragma unroll
-  for (int l = 0; l < L; ++l) {
-    if (T::LTYPE[l] != 0) {
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
-        if (k < T::LTYPE[l]) {
-          const int d = T::DADR[l] + k;
-          qf[d] = -q[T::QADR[l] + k] * dof_rec<T>(M, d)[DF_STIFFNESS] + qf[d];
-        }
-    }
-  }
-  float tau[NV];
-#pragma unroll
-  for (int d = 0; d < NV; ++d) tau[d] = 0.f;
-#pragma unroll
-  for (int a = 0; a < T::NU; ++a) {
-    const float* ac = M + T::OFF_ACT + a * ACT_STRIDE;
-    const int qa = T::ACT_Q[a], da = T::ACT_D[a];
-    const float ctl = clipf(act[a], ac[AC_CTRL_LO], ac[AC_CTRL_HI]);
-    const float b = ac[AC_GEAR] * (q[qa] * ac[AC_BIAS_Q] + qd[da] * ac[AC_BIAS_QD]);
-    float f = ac[AC_GAIN] * ctl + b;
-    f = clipf(f, ac[AC_FORCE_LO], ac[AC_FORCE_HI]);
-    tau[da] += f * ac[AC_GEAR];
-  }
-#pragma unroll
-  for (int d = 0; d < NV; ++d) qf[d] = qf[d] - bias[d] + tau[d];
-
-  // mass.matrix: CRBA on the dof-ancestor pattern (mass.py:27-83)
-  {
-    V3 kc[L];
-    Sym3 ki[L];
-    float km[L];
-#pragma unroll
-    for (int l = 0; l < L; ++l) {
-      kc[l] = cc[l], ki[l] = ci[l], km[l] = link_rec<T>(M, l)[LK_MASS];
-    }
-#pragma unroll
-    for (int l = L - 1; l >= 0; --l) {
-      const int par = T::PARENT[l];
-      if (par >= 0) {
-        kc[par] = kc[par] + kc[l];
-        ki[par] = symadd(ki[par], ki[l]);
-        km[par] = km[par] + km[l];
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-      const int l = T::DOF_LINK[i];
-      V3 fa, fv;
-      inertia_mul(kc[l], ki[l], km[l], t.ca[i], t.cv[i], fa, fv);
-      t.Lm[i][i] = dot(t.cv[i], fv) + dot(t.ca[i], fa) + dof_rec<T>(M, i)[DF_ARMATURE];
-#pragma unroll
-      for (int j = 0; j < i; ++j)
-        if ((T::DANC[i] >> j) & 1u) t.Lm[i][j] = dot(t.cv[j], fv) + dot(t.ca[j], fa);
-    }
-  }
-  // LTL factorization, leaves first
-#pragma unroll
-  for (int k = NV - 1; k >= 0; --k) {
-    const float lkk = sqrtf(t.Lm[k][k]);
-    t.Lm[k][k] = lkk;
-    t.inv[k] = 1.f / lkk;
-#pragma unroll
-    for (int j = 0; j < k; ++j)
-      if ((T::DANC[k] >> j) & 1u) t.Lm[k][j] *= t.inv[k];
-#pragma unroll
-    for (int i = 0; i < k; ++i) {
-      if ((T::DANC[k] >> i) & 1u) {
-        const float a = t.Lm[k][i];
-        t.Lm[i][i] -= a * a;
-#pragma unroll
-        for (int j = 0; j < i; ++j)
-          if ((T::DANC[i] >> j) & 1u) t.Lm[i][j] -= a * t.Lm[k][j];
-      }
-    }
-  }
-
-  // count active contacts to pick the slot capacity
   int n_act = 0;
 #pragma unroll
   for (int c = 0; c < T::NCAND; ++c) {
@@ -882,15 +958,15 @@ BGX_D int substep(const float* __restrict__ M, float* q, float* qd,
     candidate_center<T>(M, c, xp, xr, ctr, axis);
     n_act += candidate_dist<T>(M, c, ctr) < 0.f ? 1 : 0;
   }
-  int status = 0;
   if (n_act <= T::KC) {
-    solve_and_integrate<T, T::KC>(M, q, qd, t, xp, xr, cda, cdv, qf, lds, dbg);
-  } else {
-    solve_and_integrate<T, (T::NCAND > T::KC ? T::NCAND : T::KC)>(
-        M, q, qd, t, xp, xr, cda, cdv, qf, lds, dbg);
-    status |= ST_SLOW_PATH;
+    LdsRows<T::KC> rows{lds + T::L * 6 * WAVE};
+    substep_impl<T, T::KC>(M, q, qd, act, xp, xr, rc, lds, rows, dbg);
+    return 0;
   }
-  return status;
+  constexpr int KB = T::NCAND > T::KC ? T::NCAND : T::KC;
+  PrivRows<KB> rows;
+  substep_impl<T, KB>(M, q, qd, act, xp, xr, rc, lds, rows, dbg);
+  return ST_SLOW_PATH;
 }
 
 }  // namespace bgx
