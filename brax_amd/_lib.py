@@ -41,6 +41,10 @@ SIGNATURES = {
                             c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "bgx_step_probe": (c_int, [c_vp, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp,
                                c_vp]),
+    "bgx_adjoint_workspace_bytes": (c_i64, [c_vp, c_i32, c_i32, c_i32]),
+    "bgx_rollout_adjoint": (c_int, [c_vp, c_i32, c_i32, c_i32, c_i32, c_vp,
+                                    c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp,
+                                    c_vp, c_vp, c_i64, c_vp, c_vp]),
 }
 
 

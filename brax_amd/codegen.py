@@ -38,7 +38,7 @@ KIND_CAPSULE = 1
 
 # fast-path contact-slot capacity per topology (other candidates take the
 # scratch path); sized from the active-contact distribution of the workload
-DEFAULT_SLOTS = 5
+DEFAULT_SLOTS = 6
 
 JAXOPT_TOL = 1e-3  # jaxopt ProximalGradient default tol
 LS_EPS = float(np.finfo(np.float32).eps)
@@ -259,6 +259,7 @@ def registered_systems() -> List[S.System]:
       S.ant(),
       S.pendulum(1, motor_gear=150.0),
       S.pendulum(3),
+      S.humanoid(0),
   ]
 
 

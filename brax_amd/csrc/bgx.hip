@@ -1,136 +1,26 @@
-// Kernels and C-ABI entry points of the generalized-coordinates step.
+// C ABI of the generalized-coordinates step (declared in include/brax_gen.h).
 //
-// One thread per environment, 64-thread workgroups (one wavefront each).  A
-// launch advances every env by n_steps x n_frames substeps with the whole
-// state in registers; HBM traffic is the initial (q, qd), one action per env
-// step, optional per-step trajectory rows, and the final state.
+// The kernels live in one object file per topology (bgx_topo.hip); this file
+// validates arguments, owns the model handles and error strings, and
+// dispatches through the per-topology launch tables.
 #include <hip/hip_runtime.h>
 
-#include <cmath>
-#include <cstdio>
-#include <cstring>
+#include <cstdint>
 #include <string>
 
 #include "../../include/brax_gen.h"
-#include "bgx_step.h"
+#include "bgx_kernels.h"
 #include "topologies.inc"
 
-namespace bgx {
-
-struct Launch {
-  const float* M;
-  int n;
-  int n_steps;
-  int n_frames;
-  float* q;
-  float* qd;
-  const float* act;
-  long long act_stride;
-  float* q_traj;
-  float* qd_traj;
-  float* x_pos;
-  float* x_rot;
-  float* xd_ang;
-  float* xd_vel;
-  int* status;
-  float* probe;
-};
-
-template <class T>
-BGX_D void write_frames(const Launch& a, int env, const float* q,
-                        const float* qd) {
-  V3 xp[T::L], xa[T::L], xv[T::L];
-  Q4 xr[T::L];
-  forward_kinematics<T, true>(a.M, q, qd, xp, xr, xa, xv);
-#pragma unroll
-  for (int l = 0; l < T::L; ++l) {
-    const long long o3 = ((long long)env * T::L + l) * 3;
-    const long long o4 = ((long long)env * T::L + l) * 4;
-    if (a.x_pos) {
-      a.x_pos[o3] = xp[l].x, a.x_pos[o3 + 1] = xp[l].y, a.x_pos[o3 + 2] = xp[l].z;
-    }
-    if (a.x_rot) {
-      a.x_rot[o4] = xr[l].w, a.x_rot[o4 + 1] = xr[l].x;
-      a.x_rot[o4 + 2] = xr[l].y, a.x_rot[o4 + 3] = xr[l].z;
-    }
-    if (a.xd_ang) {
-      a.xd_ang[o3] = xa[l].x, a.xd_ang[o3 + 1] = xa[l].y, a.xd_ang[o3 + 2] = xa[l].z;
-    }
-    if (a.xd_vel) {
-      a.xd_vel[o3] = xv[l].x, a.xd_vel[o3 + 1] = xv[l].y, a.xd_vel[o3 + 2] = xv[l].z;
-    }
-  }
-}
-
-template <class T>
-__global__ __launch_bounds__(WAVE) void k_init(Launch a) {
-  const int env = blockIdx.x * WAVE + threadIdx.x;
-  if (env >= a.n) return;
-  float q[T::NQ], qd[T::NV];
-#pragma unroll
-  for (int k = 0; k < T::NQ; ++k) q[k] = a.q[(long long)env * T::NQ + k];
-#pragma unroll
-  for (int k = 0; k < T::NV; ++k) qd[k] = a.qd[(long long)env * T::NV + k];
-  write_frames<T>(a, env, q, qd);
-}
-
-template <class T>
-__global__ __launch_bounds__(WAVE) void k_rollout(Launch a) {
-  // link scratch (6 per link), then the fast-path slot rows (5 vectors x 4
-  // rows), which also hold a second link vector while the rows are unused
-  constexpr int kRowsArea =
-      RV_N * 4 * T::KC > T::L * 6 ? RV_N * 4 * T::KC : T::L * 6;
-  __shared__ float smem[(T::L * 6 + kRowsArea) * WAVE];
-  const int env = blockIdx.x * WAVE + threadIdx.x;
-  if (env >= a.n) return;
-  float* lds = smem + threadIdx.x;
-  float q[T::NQ], qd[T::NV];
-#pragma unroll
-  for (int k = 0; k < T::NQ; ++k) q[k] = a.q[(long long)env * T::NQ + k];
-#pragma unroll
-  for (int k = 0; k < T::NV; ++k) qd[k] = a.qd[(long long)env * T::NV + k];
-  float* probe = a.probe ? a.probe + (long long)env * (2 * T::NV + 3) : nullptr;
-  int st = 0;
-  for (int s = 0; s < a.n_steps; ++s) {
-    float act[T::NU > 0 ? T::NU : 1];
-#pragma unroll
-    for (int k = 0; k < T::NU; ++k)
-      act[k] = a.act ? a.act[s * a.act_stride + (long long)env * T::NU + k] : 0.f;
-    for (int f = 0; f < a.n_frames; ++f)
-      st |= substep<T>(a.M, q, qd, act, lds, probe);
-    if (a.q_traj) {
-#pragma unroll
-      for (int k = 0; k < T::NQ; ++k)
-        a.q_traj[((long long)s * a.n + env) * T::NQ + k] = q[k];
-    }
-    if (a.qd_traj) {
-#pragma unroll
-      for (int k = 0; k < T::NV; ++k)
-        a.qd_traj[((long long)s * a.n + env) * T::NV + k] = qd[k];
-    }
-  }
-  bool finite = true;
-#pragma unroll
-  for (int k = 0; k < T::NQ; ++k) {
-    finite = finite && isfinite(q[k]);
-    a.q[(long long)env * T::NQ + k] = q[k];
-  }
-#pragma unroll
-  for (int k = 0; k < T::NV; ++k) {
-    finite = finite && isfinite(qd[k]);
-    a.qd[(long long)env * T::NV + k] = qd[k];
-  }
-  if (!finite) st |= ST_NONFINITE;
-  if (a.x_pos || a.x_rot || a.xd_ang || a.xd_vel) write_frames<T>(a, env, q, qd);
-  if (a.status) a.status[env] = st;
-}
-
-}  // namespace bgx
-
-// ---------------------------------------------------------------------------
-// C ABI
+#define BGX_DECL_OPS(ID, T) const bgx::TopoOps* bgx_ops_##T();
+BGX_FOR_EACH_TOPOLOGY(BGX_DECL_OPS)
+#undef BGX_DECL_OPS
 
 namespace {
+
+#define BGX_OPS_PTR(ID, T) bgx_ops_##T(),
+const bgx::TopoOps* const kOps[] = {BGX_FOR_EACH_TOPOLOGY(BGX_OPS_PTR)};
+#undef BGX_OPS_PTR
 
 thread_local std::string g_err;
 
@@ -139,22 +29,34 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
-struct TopoInfo {
-  const char* name;
-  const char* sig;
-  int dims[8];
-};
-
-#define BGX_TOPO_INFO(ID, T)                                                 \
-  TopoInfo{T::NAME, T::SIG,                                                  \
-           {T::L, T::NQ, T::NV, T::NU, T::NLIM, T::NCAND, T::KC, T::SIZE}},
-const TopoInfo kTopos[] = {BGX_FOR_EACH_TOPOLOGY(BGX_TOPO_INFO)};
-#undef BGX_TOPO_INFO
-
 int check_hip(hipError_t e, const char* what) {
   if (e == hipSuccess) return BGX_OK;
   return fail(BGX_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
+
+bool valid_topo(int id) { return id >= 0 && id < BGX_NUM_TOPOLOGIES; }
+
+// fill / strided copy helpers for the adjoint driver (no memset nodes, so the
+// call can be captured in a graph)
+__global__ void k_fill(float* p, long long n, float v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_fill_i(int* p, long long n, int v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+// dst (rows, dw) <- src columns [c0, c0 + dw) of (rows, sw)
+__global__ void k_cols(float* dst, const float* src, long long rows, int dw, int sw, int c0) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < rows * dw) dst[i] = src[(i / dw) * sw + c0 + (i % dw)];
+}
+// dst (rows, w) <- src (rows, w)
+__global__ void k_copy(float* dst, const float* src, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+dim3 grid1(long long n) { return dim3((unsigned)((n + 255) / 256)); }
 
 }  // namespace
 
@@ -162,50 +64,43 @@ struct bgx_model {
   int topo;
   float* table;  // device copy of the model table
   int64_t size;
+  int maxiter;   // solver iterations (from the table header)
 };
 
 extern "C" {
 
-int bgx_version(void) { return 100; }
+int bgx_version(void) { return 200; }
 
 const char* bgx_last_error(void) { return g_err.c_str(); }
 
 int bgx_num_topologies(void) { return BGX_NUM_TOPOLOGIES; }
 
-const char* bgx_topology_name(int id) {
-  return (id >= 0 && id < BGX_NUM_TOPOLOGIES) ? kTopos[id].name : nullptr;
-}
+const char* bgx_topology_name(int id) { return valid_topo(id) ? kOps[id]->name : nullptr; }
 
-const char* bgx_topology_signature(int id) {
-  return (id >= 0 && id < BGX_NUM_TOPOLOGIES) ? kTopos[id].sig : nullptr;
-}
+const char* bgx_topology_signature(int id) { return valid_topo(id) ? kOps[id]->sig : nullptr; }
 
 int bgx_topology_dims(int id, int32_t* dims) {
-  if (id < 0 || id >= BGX_NUM_TOPOLOGIES)
-    return fail(BGX_ERR_TOPOLOGY, "unknown topology id");
+  if (!valid_topo(id)) return fail(BGX_ERR_TOPOLOGY, "unknown topology id");
   if (!dims) return fail(BGX_ERR_ARG, "dims is null");
-  for (int k = 0; k < 8; ++k) dims[k] = kTopos[id].dims[k];
+  for (int k = 0; k < 8; ++k) dims[k] = kOps[id]->dims[k];
   return BGX_OK;
 }
 
-int bgx_model_create(int id, const float* host_table, int64_t n_floats,
-                     bgx_model** out) {
-  if (id < 0 || id >= BGX_NUM_TOPOLOGIES)
-    return fail(BGX_ERR_TOPOLOGY, "unknown topology id");
+int bgx_model_create(int id, const float* host_table, int64_t n_floats, bgx_model** out) {
+  if (!valid_topo(id)) return fail(BGX_ERR_TOPOLOGY, "unknown topology id");
   if (!host_table || !out) return fail(BGX_ERR_ARG, "null argument");
-  if (n_floats != kTopos[id].dims[7])
+  if (n_floats != kOps[id]->dims[7])
     return fail(BGX_ERR_ARG, "model table size does not match the topology");
   float* d = nullptr;
   int rc = check_hip(hipMalloc(&d, n_floats * sizeof(float)), "hipMalloc");
   if (rc) return rc;
-  rc = check_hip(hipMemcpy(d, host_table, n_floats * sizeof(float),
-                           hipMemcpyHostToDevice),
+  rc = check_hip(hipMemcpy(d, host_table, n_floats * sizeof(float), hipMemcpyHostToDevice),
                  "hipMemcpy");
   if (rc) {
     (void)hipFree(d);
     return rc;
   }
-  *out = new bgx_model{id, d, n_floats};
+  *out = new bgx_model{id, d, n_floats, (int)host_table[bgx::H_MAXITER]};
   return BGX_OK;
 }
 
@@ -218,36 +113,21 @@ int bgx_model_destroy(bgx_model* m) {
 
 int bgx_model_topology(const bgx_model* m) { return m ? m->topo : -1; }
 
-static int launch(const bgx_model* m, bool init_only, bgx::Launch a,
-                  void* stream) {
+static int launch(const bgx_model* m, bool init_only, bgx::Launch a, void* stream) {
   if (!m) return fail(BGX_ERR_ARG, "model is null");
+  if (!valid_topo(m->topo)) return fail(BGX_ERR_TOPOLOGY, "unknown topology id");
   if (a.n < 0) return fail(BGX_ERR_ARG, "negative env count");
-  if (a.n_steps < 0 || a.n_frames < 0)
-    return fail(BGX_ERR_ARG, "negative step count");
+  if (a.n_steps < 0 || a.n_frames < 0) return fail(BGX_ERR_ARG, "negative step count");
   if (a.n == 0) return BGX_OK;
   if (!a.q || !a.qd) return fail(BGX_ERR_ARG, "q/qd must be non-null");
   a.M = m->table;
-  const dim3 grid((a.n + bgx::WAVE - 1) / bgx::WAVE), block(bgx::WAVE);
+  const bgx::TopoOps* ops = kOps[m->topo];
   hipStream_t s = static_cast<hipStream_t>(stream);
-  switch (m->topo) {
-#define BGX_CASE(ID, T)                                                \
-  case ID:                                                             \
-    if (init_only)                                                     \
-      hipLaunchKernelGGL(bgx::k_init<T>, grid, block, 0, s, a);        \
-    else                                                               \
-      hipLaunchKernelGGL(bgx::k_rollout<T>, grid, block, 0, s, a);     \
-    break;
-    BGX_FOR_EACH_TOPOLOGY(BGX_CASE)
-#undef BGX_CASE
-    default:
-      return fail(BGX_ERR_TOPOLOGY, "unknown topology id");
-  }
-  return check_hip(hipGetLastError(), "kernel launch");
+  return check_hip(init_only ? ops->init(a, s) : ops->rollout(a, s), "kernel launch");
 }
 
-int bgx_init(const bgx_model* m, int32_t n, const float* q, const float* qd,
-             float* x_pos, float* x_rot, float* xd_ang, float* xd_vel,
-             void* stream) {
+int bgx_init(const bgx_model* m, int32_t n, const float* q, const float* qd, float* x_pos,
+             float* x_rot, float* xd_ang, float* xd_vel, void* stream) {
   bgx::Launch a{};
   a.n = n;
   a.q = const_cast<float*>(q);
@@ -256,9 +136,9 @@ int bgx_init(const bgx_model* m, int32_t n, const float* q, const float* qd,
   return launch(m, true, a, stream);
 }
 
-int bgx_step(const bgx_model* m, int32_t n, int32_t n_frames, float* q,
-             float* qd, const float* act, float* x_pos, float* x_rot,
-             float* xd_ang, float* xd_vel, int32_t* status, void* stream) {
+int bgx_step(const bgx_model* m, int32_t n, int32_t n_frames, float* q, float* qd,
+             const float* act, float* x_pos, float* x_rot, float* xd_ang, float* xd_vel,
+             int32_t* status, void* stream) {
   bgx::Launch a{};
   a.n = n, a.n_steps = 1, a.n_frames = n_frames;
   a.q = q, a.qd = qd, a.act = act, a.act_stride = 0;
@@ -267,10 +147,9 @@ int bgx_step(const bgx_model* m, int32_t n, int32_t n_frames, float* q,
   return launch(m, false, a, stream);
 }
 
-int bgx_rollout(const bgx_model* m, int32_t n, int32_t n_steps,
-                int32_t n_frames, float* q, float* qd, const float* act,
-                int64_t act_step_stride, float* q_traj, float* qd_traj,
-                float* x_pos, float* x_rot, float* xd_ang, float* xd_vel,
+int bgx_rollout(const bgx_model* m, int32_t n, int32_t n_steps, int32_t n_frames, float* q,
+                float* qd, const float* act, int64_t act_step_stride, float* q_traj,
+                float* qd_traj, float* x_pos, float* x_rot, float* xd_ang, float* xd_vel,
                 int32_t* status, void* stream) {
   bgx::Launch a{};
   a.n = n, a.n_steps = n_steps, a.n_frames = n_frames;
@@ -281,14 +160,130 @@ int bgx_rollout(const bgx_model* m, int32_t n, int32_t n_steps,
   return launch(m, false, a, stream);
 }
 
-int bgx_step_probe(const bgx_model* m, int32_t n, int32_t n_frames, float* q,
-                   float* qd, const float* act, float* probe, void* stream) {
+int bgx_step_probe(const bgx_model* m, int32_t n, int32_t n_frames, float* q, float* qd,
+                   const float* act, float* probe, void* stream) {
   if (!probe) return fail(BGX_ERR_ARG, "probe is null");
   bgx::Launch a{};
   a.n = n, a.n_steps = 1, a.n_frames = n_frames;
   a.q = q, a.qd = qd, a.act = act, a.act_stride = 0;
   a.probe = probe;
   return launch(m, false, a, stream);
+}
+
+// ---------------------------------------------------------------------------
+// adjoint of a rollout
+
+namespace {
+
+struct AdjWorkspace {
+  float* sub_q;   // (k * n_frames, n, NQ)
+  float* sub_qd;  // (k * n_frames, n, NV)
+  float* q;       // (n, NQ) segment start, advanced by the recompute
+  float* qd;      // (n, NV)
+  float* adj;     // (n, NQ + NV)
+  float* rec;     // (rec, n)
+  int64_t bytes;
+};
+
+int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
+
+AdjWorkspace adj_layout(const bgx::TopoOps* ops, float* base, int64_t n, int64_t k,
+                        int64_t n_frames) {
+  const int64_t nq = ops->dims[1], nv = ops->dims[2];
+  AdjWorkspace w{};
+  char* p = reinterpret_cast<char*>(base);
+  int64_t off = 0;
+  auto take = [&](int64_t floats) {
+    float* r = base ? reinterpret_cast<float*>(p + off) : nullptr;
+    off += align256(floats * (int64_t)sizeof(float));
+    return r;
+  };
+  w.sub_q = take(k * n_frames * n * nq);
+  w.sub_qd = take(k * n_frames * n * nv);
+  w.q = take(n * nq);
+  w.qd = take(n * nv);
+  w.adj = take(n * (nq + nv));
+  w.rec = take((int64_t)ops->adj_rec * n);
+  w.bytes = off;
+  return w;
+}
+
+}  // namespace
+
+int64_t bgx_adjoint_workspace_bytes(const bgx_model* m, int32_t n, int32_t ckpt_every,
+                                    int32_t n_frames) {
+  if (!m || !valid_topo(m->topo) || n < 0 || ckpt_every < 1 || n_frames < 1) return -1;
+  return adj_layout(kOps[m->topo], nullptr, n, ckpt_every, n_frames).bytes;
+}
+
+int bgx_rollout_adjoint(const bgx_model* m, int32_t n, int32_t n_steps, int32_t n_frames,
+                        int32_t ckpt_every, const float* ckpt_q, const float* ckpt_qd,
+                        const float* act, int64_t act_step_stride, const float* gq_traj,
+                        const float* gqd_traj, float* gact, float* gq0, float* gqd0,
+                        float* workspace, int64_t workspace_bytes, int32_t* status,
+                        void* stream) {
+  if (!m) return fail(BGX_ERR_ARG, "model is null");
+  if (!valid_topo(m->topo)) return fail(BGX_ERR_TOPOLOGY, "unknown topology id");
+  if (n < 0 || n_steps < 0 || n_frames < 1 || ckpt_every < 1)
+    return fail(BGX_ERR_ARG, "bad env, step, frame or checkpoint count");
+  if (m->maxiter > bgx::ADJ_MAXITER)
+    return fail(BGX_ERR_ARG, "the adjoint unrolls at most " +
+                                 std::to_string(bgx::ADJ_MAXITER) + " solver iterations");
+  if (n == 0 || n_steps == 0) return BGX_OK;
+  if (!ckpt_q || !ckpt_qd || !gq0 || !gqd0 || !workspace)
+    return fail(BGX_ERR_ARG, "null checkpoint, output or workspace pointer");
+  const bgx::TopoOps* ops = kOps[m->topo];
+  const int nq = ops->dims[1], nv = ops->dims[2], nu = ops->dims[3];
+  if (nu > 0 && !gact) return fail(BGX_ERR_ARG, "gact is null");
+  const AdjWorkspace w = adj_layout(ops, workspace, n, ckpt_every, n_frames);
+  if (workspace_bytes < w.bytes) return fail(BGX_ERR_ARG, "workspace too small");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int rc;
+  const long long nact = (long long)n_steps * n * nu, nadj = (long long)n * (nq + nv);
+  if (nu > 0) hipLaunchKernelGGL(k_fill, grid1(nact), dim3(256), 0, s, gact, nact, 0.f);
+  hipLaunchKernelGGL(k_fill, grid1(nadj), dim3(256), 0, s, w.adj, nadj, 0.f);
+  if (status) hipLaunchKernelGGL(k_fill_i, grid1(n), dim3(256), 0, s, status, (long long)n, 0);
+  if ((rc = check_hip(hipGetLastError(), "fill"))) return rc;
+
+  const int n_seg = (n_steps + ckpt_every - 1) / ckpt_every;
+  for (int seg = n_seg - 1; seg >= 0; --seg) {
+    const int s0 = seg * ckpt_every;
+    const int ns = n_steps - s0 < ckpt_every ? n_steps - s0 : ckpt_every;
+    // recompute the states at the start of every substep of the segment
+    hipLaunchKernelGGL(k_copy, grid1((long long)n * nq), dim3(256), 0, s, w.q,
+                       ckpt_q + (long long)seg * n * nq, (long long)n * nq);
+    hipLaunchKernelGGL(k_copy, grid1((long long)n * nv), dim3(256), 0, s, w.qd,
+                       ckpt_qd + (long long)seg * n * nv, (long long)n * nv);
+    bgx::Launch f{};
+    f.M = m->table, f.n = n, f.n_steps = ns, f.n_frames = n_frames;
+    f.q = w.q, f.qd = w.qd;
+    f.act = act ? act + (long long)s0 * act_step_stride : nullptr;
+    f.act_stride = act_step_stride;
+    f.sub_q = w.sub_q, f.sub_qd = w.sub_qd;
+    if ((rc = check_hip(ops->rollout(f, s), "recompute launch"))) return rc;
+    // substeps of the segment, last first
+    for (int i = ns * n_frames - 1; i >= 0; --i) {
+      const int t = s0 + i / n_frames;
+      const bool step_end = (i % n_frames) == n_frames - 1;
+      bgx::AdjLaunch a{};
+      a.M = m->table, a.n = n;
+      a.q = w.sub_q + (long long)i * n * nq;
+      a.qd = w.sub_qd + (long long)i * n * nv;
+      a.act = act ? act + (long long)t * act_step_stride : nullptr;
+      a.gq_add = (step_end && gq_traj) ? gq_traj + (long long)t * n * nq : nullptr;
+      a.gqd_add = (step_end && gqd_traj) ? gqd_traj + (long long)t * n * nv : nullptr;
+      a.adj = w.adj, a.rec = w.rec;
+      a.gact = gact ? gact + (long long)t * n * nu : nullptr;
+      a.status = status;
+      if ((rc = check_hip(ops->adj_a(a, s), "adjoint stage A launch"))) return rc;
+      if ((rc = check_hip(ops->adj_b(a, s), "adjoint stage B launch"))) return rc;
+    }
+  }
+  hipLaunchKernelGGL(k_cols, grid1((long long)n * nq), dim3(256), 0, s, gq0, w.adj,
+                     (long long)n, nq, nq + nv, 0);
+  hipLaunchKernelGGL(k_cols, grid1((long long)n * nv), dim3(256), 0, s, gqd0, w.adj,
+                     (long long)n, nv, nq + nv, nq);
+  return check_hip(hipGetLastError(), "output copy");
 }
 
 }  // extern "C"

@@ -1,0 +1,243 @@
+// Kernels of the generalized-coordinates step and its adjoint, and the
+// per-topology launch table.  bgx_topo.hip instantiates them for one
+// topology per object file; bgx.hip holds the C ABI and dispatches through
+// the tables.
+//
+// Step kernels: one thread per environment, 64-thread workgroups (one
+// wavefront each).  A launch advances every env by n_steps x n_frames
+// substeps with the whole state in registers; HBM traffic is the initial
+// (q, qd), one action per env step, optional per-step trajectory rows, and
+// the final state.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "bgx_adjoint.h"
+#include "bgx_step.h"
+
+namespace bgx {
+
+struct Launch {
+  const float* M;
+  int n;
+  int n_steps;
+  int n_frames;
+  float* q;
+  float* qd;
+  const float* act;
+  long long act_stride;
+  float* q_traj;
+  float* qd_traj;
+  float* x_pos;
+  float* x_rot;
+  float* xd_ang;
+  float* xd_vel;
+  int* status;
+  float* probe;
+  float* sub_q;   // (n_steps * n_frames, n, NQ): state at the start of each substep
+  float* sub_qd;  // (n_steps * n_frames, n, NV)
+};
+
+// one adjoint substep: state (q, qd) of substep i, its action, the running
+// cotangent adj (n, NQ + NV) of the substep's output state
+struct AdjLaunch {
+  const float* M;
+  int n;
+  const float* q;    // (n, NQ)
+  const float* qd;   // (n, NV)
+  const float* act;  // (n, NU) or null
+  const float* gq_add;   // (n, NQ) cotangent added to adj's q part, or null
+  const float* gqd_add;  // (n, NV)
+  float* adj;        // (n, NQ + NV), read by stage A, written by stage B
+  float* rec;        // (SIZE, n) stage A record
+  float* gact;       // (n, NU) accumulated
+  int* status;
+};
+
+template <class T>
+BGX_D void write_frames(const Launch& a, int env, const float* q, const float* qd) {
+  V3 xp[T::L], xa[T::L], xv[T::L];
+  Q4 xr[T::L];
+  forward_kinematics<T, true>(a.M, q, qd, xp, xr, xa, xv);
+#pragma unroll
+  for (int l = 0; l < T::L; ++l) {
+    const long long o3 = ((long long)env * T::L + l) * 3;
+    const long long o4 = ((long long)env * T::L + l) * 4;
+    if (a.x_pos) {
+      a.x_pos[o3] = xp[l].x, a.x_pos[o3 + 1] = xp[l].y, a.x_pos[o3 + 2] = xp[l].z;
+    }
+    if (a.x_rot) {
+      a.x_rot[o4] = xr[l].w, a.x_rot[o4 + 1] = xr[l].x;
+      a.x_rot[o4 + 2] = xr[l].y, a.x_rot[o4 + 3] = xr[l].z;
+    }
+    if (a.xd_ang) {
+      a.xd_ang[o3] = xa[l].x, a.xd_ang[o3 + 1] = xa[l].y, a.xd_ang[o3 + 2] = xa[l].z;
+    }
+    if (a.xd_vel) {
+      a.xd_vel[o3] = xv[l].x, a.xd_vel[o3 + 1] = xv[l].y, a.xd_vel[o3 + 2] = xv[l].z;
+    }
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(WAVE) void k_init(Launch a) {
+  const int env = blockIdx.x * WAVE + threadIdx.x;
+  if (env >= a.n) return;
+  float q[T::NQ], qd[T::NV];
+#pragma unroll
+  for (int k = 0; k < T::NQ; ++k) q[k] = a.q[(long long)env * T::NQ + k];
+#pragma unroll
+  for (int k = 0; k < T::NV; ++k) qd[k] = a.qd[(long long)env * T::NV + k];
+  write_frames<T>(a, env, q, qd);
+}
+
+template <class T>
+__global__ __launch_bounds__(WAVE) void k_rollout(Launch a) {
+  // link scratch (6 per link), then the fast-path slot rows (4 vectors x 4
+  // rows), which also hold a second link vector while the rows are unused
+  constexpr int kRowsArea =
+      RV_N * 4 * T::KC > T::L * 6 ? RV_N * 4 * T::KC : T::L * 6;
+  __shared__ float smem[(T::L * 6 + kRowsArea) * WAVE];
+  const int env = blockIdx.x * WAVE + threadIdx.x;
+  if (env >= a.n) return;
+  float* lds = smem + threadIdx.x;
+  float q[T::NQ], qd[T::NV];
+#pragma unroll
+  for (int k = 0; k < T::NQ; ++k) q[k] = a.q[(long long)env * T::NQ + k];
+#pragma unroll
+  for (int k = 0; k < T::NV; ++k) qd[k] = a.qd[(long long)env * T::NV + k];
+  float* probe = a.probe ? a.probe + (long long)env * (2 * T::NV + 3) : nullptr;
+  int st = 0;
+  for (int s = 0; s < a.n_steps; ++s) {
+    float act[T::NU > 0 ? T::NU : 1];
+#pragma unroll
+    for (int k = 0; k < T::NU; ++k)
+      act[k] = a.act ? a.act[s * a.act_stride + (long long)env * T::NU + k] : 0.f;
+    for (int f = 0; f < a.n_frames; ++f) {
+      if (a.sub_q) {
+        const long long i = (long long)s * a.n_frames + f;
+#pragma unroll
+        for (int k = 0; k < T::NQ; ++k) a.sub_q[(i * a.n + env) * T::NQ + k] = q[k];
+#pragma unroll
+        for (int k = 0; k < T::NV; ++k) a.sub_qd[(i * a.n + env) * T::NV + k] = qd[k];
+      }
+      st |= substep<T>(a.M, q, qd, act, lds, probe);
+    }
+    if (a.q_traj) {
+#pragma unroll
+      for (int k = 0; k < T::NQ; ++k)
+        a.q_traj[((long long)s * a.n + env) * T::NQ + k] = q[k];
+    }
+    if (a.qd_traj) {
+#pragma unroll
+      for (int k = 0; k < T::NV; ++k)
+        a.qd_traj[((long long)s * a.n + env) * T::NV + k] = qd[k];
+    }
+  }
+  bool finite = true;
+#pragma unroll
+  for (int k = 0; k < T::NQ; ++k) {
+    finite = finite && isfinite(q[k]);
+    a.q[(long long)env * T::NQ + k] = q[k];
+  }
+#pragma unroll
+  for (int k = 0; k < T::NV; ++k) {
+    finite = finite && isfinite(qd[k]);
+    a.qd[(long long)env * T::NV + k] = qd[k];
+  }
+  if (!finite) st |= ST_NONFINITE;
+  if (a.x_pos || a.x_rot || a.xd_ang || a.xd_vel) write_frames<T>(a, env, q, qd);
+  if (a.status) a.status[env] = st;
+}
+
+// adjoint stage A: one thread per env
+template <class T>
+__global__ __launch_bounds__(WAVE) void k_adj_a(AdjLaunch a) {
+  constexpr int NQ = T::NQ, NV = T::NV;
+  const int env = blockIdx.x * WAVE + threadIdx.x;
+  if (env >= a.n) return;
+  float q[NQ], qd[NV], gq1[NQ], gqd1[NV];
+  float act[T::NU > 0 ? T::NU : 1], gact[T::NU > 0 ? T::NU : 1];
+  const float* adj = a.adj + (long long)env * (NQ + NV);
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) {
+    q[k] = a.q[(long long)env * NQ + k];
+    gq1[k] = adj[k] + (a.gq_add ? a.gq_add[(long long)env * NQ + k] : 0.f);
+  }
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    qd[k] = a.qd[(long long)env * NV + k];
+    gqd1[k] = adj[NQ + k] + (a.gqd_add ? a.gqd_add[(long long)env * NV + k] : 0.f);
+  }
+#pragma unroll
+  for (int k = 0; k < T::NU; ++k) {
+    act[k] = a.act ? a.act[(long long)env * T::NU + k] : 0.f;
+    gact[k] = 0.f;
+  }
+  const int st = adjoint_a<T>(a.M, q, qd, act, gq1, gqd1, RecPtr{a.rec + env, a.n}, gact);
+#pragma unroll
+  for (int k = 0; k < T::NU; ++k) a.gact[(long long)env * T::NU + k] += gact[k];
+  if (a.status) a.status[env] |= (st < 0 ? ST_NONFINITE : st);
+}
+
+// adjoint stage B: one thread per (env, input coordinate)
+template <class T>
+__global__ __launch_bounds__(WAVE) void k_adj_b(AdjLaunch a) {
+  constexpr int NQ = T::NQ, NV = T::NV, ND = NQ + NV;
+  const long long g = (long long)blockIdx.x * WAVE + threadIdx.x;
+  if (g >= (long long)a.n * ND) return;
+  const int env = (int)(g / ND), dir = (int)(g % ND);
+  float act[T::NU > 0 ? T::NU : 1];
+#pragma unroll
+  for (int k = 0; k < T::NU; ++k) act[k] = a.act ? a.act[(long long)env * T::NU + k] : 0.f;
+  const float r = adjoint_b<T>(a.M, a.q + (long long)env * NQ, a.qd + (long long)env * NV,
+                               act, RecPtr{a.rec + env, a.n}, dir);
+  a.adj[(long long)env * ND + dir] = r;
+}
+
+// per-topology host entry points
+
+struct TopoOps {
+  const char* name;
+  const char* sig;
+  int dims[8];  // L, NQ, NV, NU, NLIM, NCAND, KC, table size
+  int adj_rec;  // floats per env of the adjoint record
+  hipError_t (*init)(const Launch&, hipStream_t);
+  hipError_t (*rollout)(const Launch&, hipStream_t);
+  hipError_t (*adj_a)(const AdjLaunch&, hipStream_t);
+  hipError_t (*adj_b)(const AdjLaunch&, hipStream_t);
+};
+
+template <class T>
+struct Ops {
+  static hipError_t init(const Launch& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_init<T>, dim3((a.n + WAVE - 1) / WAVE), dim3(WAVE), 0, s, a);
+    return hipGetLastError();
+  }
+  static hipError_t rollout(const Launch& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_rollout<T>, dim3((a.n + WAVE - 1) / WAVE), dim3(WAVE), 0, s, a);
+    return hipGetLastError();
+  }
+  static hipError_t adj_a(const AdjLaunch& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_adj_a<T>, dim3((a.n + WAVE - 1) / WAVE), dim3(WAVE), 0, s, a);
+    return hipGetLastError();
+  }
+  static hipError_t adj_b(const AdjLaunch& a, hipStream_t s) {
+    const long long threads = (long long)a.n * (T::NQ + T::NV);
+    hipLaunchKernelGGL(k_adj_b<T>, dim3((unsigned)((threads + WAVE - 1) / WAVE)), dim3(WAVE),
+                       0, s, a);
+    return hipGetLastError();
+  }
+  static TopoOps table() {
+    return TopoOps{T::NAME,
+                   T::SIG,
+                   {T::L, T::NQ, T::NV, T::NU, T::NLIM, T::NCAND, T::KC, T::SIZE},
+                   AdjRec<T>::SIZE,
+                   &init,
+                   &rollout,
+                   &adj_a,
+                   &adj_b};
+  }
+};
+
+}  // namespace bgx

@@ -98,46 +98,57 @@ constexpr bool hosts_contact(int l) {
 // ---------------------------------------------------------------------------
 // kinematics.forward (kinematics.py:31-108); XD: also compute xd
 
-template <class T, bool XD>
-BGX_D void forward_kinematics(const float* __restrict__ M, const float* q,
-                              const float* qd, V3* xp, Q4* xr, V3* xa, V3* xv) {
-  V3 jp[T::L], ja[T::L], jv[T::L];
-  Q4 jr[T::L];
+template <class T, bool XD, class S>
+BGX_D void forward_kinematics(const float* __restrict__ M, const S* q, const S* qd,
+                              V3T<S>* xp, Q4T<S>* xr, V3T<S>* xa, V3T<S>* xv) {
+  V3T<S> jp[T::L], ja[T::L], jv[T::L];
+  Q4T<S> jr[T::L];
+  const S zero = lift<S>(0.f);
 #pragma unroll
   for (int l = 0; l < T::L; ++l) {
     const float* lk = link_rec<T>(M, l);
     const int qa = T::QADR[l], da = T::DADR[l];
-    V3 p = v3zero(), a = v3zero(), v = v3zero();
-    Q4 r = qident();
+    V3T<S> p{zero, zero, zero}, a{zero, zero, zero}, v{zero, zero, zero};
+    Q4T<S> r = lift4<S>(qident());
     if (T::LTYPE[l] == 0) {  // free joint (kinematics.py:48-51)
-      p = v3(q[qa], q[qa + 1], q[qa + 2]);
-      r = Q4{q[qa + 3], q[qa + 4], q[qa + 5], q[qa + 6]};
-      a = v3(qd[da + 3], qd[da + 4], qd[da + 5]);
-      v = v3(qd[da], qd[da + 1], qd[da + 2]);
+      p = V3T<S>{q[qa], q[qa + 1], q[qa + 2]};
+      r = Q4T<S>{q[qa + 3], q[qa + 4], q[qa + 5], q[qa + 6]};
+      if (XD) {
+        a = V3T<S>{qd[da + 3], qd[da + 4], qd[da + 5]};
+        v = V3T<S>{qd[da], qd[da + 1], qd[da + 2]};
+      }
     } else {  // 1-3 stacked axis joints (kinematics.py:52-81)
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         if (k < T::LTYPE[l]) {
           const V3 ang = dof_ang<T>(M, da + k), vel = dof_vel<T>(M, da + k);
-          const float qk = q[qa + k], qdk = qd[da + k];
-          const Q4 rk = qnormalize(qrot_axis(ang, qk));
-          const V3 pk = vel * qk, jda = ang * qdk, jdv = vel * qdk;
+          const S qk = q[qa + k];
+          const Q4T<S> rk = qnormalize(qrot_axis(lift3<S>(ang), qk));
+          const V3T<S> pk = vel * qk;
           if (k == 0) {
-            p = pk, r = rk, a = jda, v = jdv;
+            p = pk, r = rk;
           } else {
             p = p + rotate(pk, r);
             r = qmul(r, rk);
-            a = a + rotate(jda, rk);
-            v = v + rotate(jdv + cross(pk, jda), rk);
+          }
+          if (XD) {
+            const S qdk = qd[da + k];
+            const V3T<S> jda = ang * qdk, jdv = vel * qdk;
+            if (k == 0) {
+              a = jda, v = jdv;
+            } else {
+              a = a + rotate(jda, rk);
+              v = v + rotate(jdv + cross(pk, jda), rk);
+            }
           }
         }
       }
     }
     // joint anchor offset and link transform (kinematics.py:85-87)
-    const V3 jpos = ldv3(lk + LK_JPOS);
+    const V3T<S> jpos = lift3<S>(ldv3(lk + LK_JPOS));
     p = (p + jpos) - rotate(jpos, r);
-    const V3 tpos = ldv3(lk + LK_TPOS);
-    const Q4 trot = ldq4(lk + LK_TROT);
+    const V3T<S> tpos = lift3<S>(ldv3(lk + LK_TPOS));
+    const Q4T<S> trot = lift4<S>(ldq4(lk + LK_TROT));
     p = tpos + rotate(p, trot);
     r = qmul(trot, r);
     jp[l] = p, jr[l] = r, ja[l] = a, jv[l] = v;
@@ -157,7 +168,7 @@ BGX_D void forward_kinematics(const float* __restrict__ M, const float* q,
       xp[l] = xp[par] + rotate(jp[l], xr[par]);
       xr[l] = qmul(xr[par], jr[l]);
       if (XD) {
-        V3 vel = xv[par] + cross(xa[par], xp[l] - xp[par]);
+        V3T<S> vel = xv[par] + cross(xa[par], xp[l] - xp[par]);
         xv[l] = vel + rotate(jv[l], xr[par]);
         xa[l] = xa[par] + rotate(ja[l], xr[l]);
       }
@@ -178,6 +189,31 @@ struct Tree {
   V3 rc[T::L];              // root COM of each link's tree
 };
 
+// M = L^T L in place on the lower part of M (Featherstone's LTL: leaves
+// first, no fill-in outside the dof-ancestor pattern)
+template <class T>
+BGX_D void ltl_factor(Tree<T>& t) {
+#pragma unroll
+  for (int k = T::NV - 1; k >= 0; --k) {
+    const float lkk = fsqrt(t.Lm[k][k]);
+    t.Lm[k][k] = lkk;
+    t.inv[k] = rcp(lkk);
+#pragma unroll
+    for (int j = 0; j < k; ++j)
+      if ((T::DANC[k] >> j) & 1u) t.Lm[k][j] *= t.inv[k];
+#pragma unroll
+    for (int i = 0; i < k; ++i) {
+      if ((T::DANC[k] >> i) & 1u) {
+        const float a = t.Lm[k][i];
+        t.Lm[i][i] -= a * a;
+#pragma unroll
+        for (int j = 0; j < i; ++j)
+          if ((T::DANC[i] >> j) & 1u) t.Lm[i][j] -= a * t.Lm[k][j];
+      }
+    }
+  }
+}
+
 // M z = tau with M = L^T L (in place on z)
 template <class T>
 BGX_D void ltl_solve(const Tree<T>& t, float* z) {
@@ -197,45 +233,63 @@ BGX_D void ltl_solve(const Tree<T>& t, float* z) {
   }
 }
 
+// dofs of link l and of its ancestors (bitmask)
+template <class T>
+constexpr unsigned link_dof_mask(int l) {
+  const int nd = T::LTYPE[l] == 0 ? 6 : T::LTYPE[l];
+  const int last = T::DADR[l] + nd - 1;
+  return T::DANC[last] | (1u << last);
+}
+
 // spatial velocity of every link from generalized velocities z (the forward
 // scan of dynamics.py:93-103 with cdof)
-template <class T>
-BGX_D void link_velocities(const Tree<T>& t, const float* z, V3* va, V3* vv) {
+template <class T, class S, class Z>
+BGX_D void link_vel(const V3T<S>* ca, const V3T<S>* cv, const Z* z, V3T<S>* va,
+                    V3T<S>* vv) {
+  const S zero = lift<S>(0.f);
+  const V3T<S> z3{zero, zero, zero};
 #pragma unroll
   for (int l = 0; l < T::L; ++l) {
     const int par = T::PARENT[l];
-    V3 a = par < 0 ? v3zero() : va[par];
-    V3 v = par < 0 ? v3zero() : vv[par];
+    V3T<S> a = par < 0 ? z3 : va[par];
+    V3T<S> v = par < 0 ? z3 : vv[par];
     const int nd = T::LTYPE[l] == 0 ? 6 : T::LTYPE[l];
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
       if (k < nd) {
         const int d = T::DADR[l] + k;
-        a = a + t.ca[d] * z[d];
-        v = v + t.cv[d] * z[d];
+        a = a + ca[d] * z[d];
+        v = v + cv[d] * z[d];
       }
     }
     va[l] = a, vv[l] = v;
   }
 }
+template <class T>
+BGX_D void link_velocities(const Tree<T>& t, const float* z, V3* va, V3* vv) {
+  link_vel<T>(t.ca, t.cv, z, va, vv);
+}
 
 // generalized force of per-link spatial forces (fa: torque about the root
 // COM, fv: force), accumulated leaf to root (dynamics.py:174-182); each
 // link's dofs are projected as soon as its subtree sum is final
-template <class T>
-BGX_D void project_forces(const Tree<T>& t, const V3* fa, const V3* fv, float* tau) {
-  V3 acc_a[T::L], acc_v[T::L];
+template <class T, class S>
+BGX_D void project_forces_on(const V3T<S>* ca, const V3T<S>* cv, const V3T<S>* fa,
+                             const V3T<S>* fv, S* tau) {
+  const S zero = lift<S>(0.f);
+  V3T<S> acc_a[T::L], acc_v[T::L];
 #pragma unroll
-  for (int l = 0; l < T::L; ++l) acc_a[l] = v3zero(), acc_v[l] = v3zero();
+  for (int l = 0; l < T::L; ++l)
+    acc_a[l] = V3T<S>{zero, zero, zero}, acc_v[l] = V3T<S>{zero, zero, zero};
 #pragma unroll
   for (int l = T::L - 1; l >= 0; --l) {
-    const V3 a = fa[l] + acc_a[l], v = fv[l] + acc_v[l];
+    const V3T<S> a = fa[l] + acc_a[l], v = fv[l] + acc_v[l];
     const int nd = T::LTYPE[l] == 0 ? 6 : T::LTYPE[l];
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
       if (k < nd) {
         const int d = T::DADR[l] + k;
-        tau[d] = dot(t.cv[d], v) + dot(t.ca[d], a);
+        tau[d] = dot(cv[d], v) + dot(ca[d], a);
       }
     }
     const int par = T::PARENT[l];
@@ -245,58 +299,346 @@ BGX_D void project_forces(const Tree<T>& t, const V3* fa, const V3* fv, float* t
     }
   }
 }
+template <class T>
+BGX_D void project_forces(const Tree<T>& t, const V3* fa, const V3* fv, float* tau) {
+  project_forces_on<T>(t.ca, t.cv, fa, fv, tau);
+}
+
+// ---------------------------------------------------------------------------
+// dynamics pieces, generic in the scalar (float for the step, Dual for the
+// adjoint's tangent pass)
+
+// root COM of the tree (dynamics.py:37-43)
+template <class T, class S>
+BGX_D V3T<S> root_com(const float* __restrict__ M, const V3T<S>* xp, const Q4T<S>* xr) {
+  const S zero = lift<S>(0.f);
+  V3T<S> sm{zero, zero, zero};
+  float ms = 0.f;
+#pragma unroll
+  for (int l = 0; l < T::L; ++l) {
+    const float* lk = link_rec<T>(M, l);
+    const float m = lk[LK_MASS];
+    sm = sm + (xp[l] + rotate(lift3<S>(ldv3(lk + LK_IPOS)), xr[l])) * m;
+    ms = ms + m;
+  }
+  const float ims = rcp(ms);
+  return V3T<S>{sm.x * ims, sm.y * ims, sm.z * ims};
+}
+
+// cinr (dynamics.py:44-55): per link, the mass-weighted COM offset from the
+// root COM cc = m (xi - rc) and the rotational inertia about the root COM ci
+template <class T, class S>
+BGX_D void com_inertia(const float* __restrict__ M, const V3T<S>* xp, const Q4T<S>* xr,
+                       V3T<S> rc, V3T<S>* cc, Sym3T<S>* ci) {
+#pragma unroll
+  for (int l = 0; l < T::L; ++l) {
+    const float* lk = link_rec<T>(M, l);
+    const V3T<S> xip = xp[l] + rotate(lift3<S>(ldv3(lk + LK_IPOS)), xr[l]);
+    const Q4T<S> xir = qmul(xr[l], lift4<S>(ldq4(lk + LK_IROT)));
+    const float m = lk[LK_MASS];
+    const V3T<S> p = xip - rc;
+    S R[3][3];
+    quat_to_3x3(xir, R);
+    const float* i0 = lk + LK_I;
+    const Sym3T<S> I0{lift<S>(i0[0]), lift<S>(i0[1]), lift<S>(i0[2]),
+                      lift<S>(i0[3]), lift<S>(i0[4]), lift<S>(i0[5])};
+    S RI[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const V3T<S> c = symmul(I0, V3T<S>{R[i][0], R[i][1], R[i][2]});
+      RI[i][0] = c.x, RI[i][1] = c.y, RI[i][2] = c.z;
+    }
+    auto rir = [&](int i, int j) {
+      return RI[i][0] * R[j][0] + RI[i][1] * R[j][1] + RI[i][2] * R[j][2];
+    };
+    const S pp = dot(p, p);
+    ci[l] = Sym3T<S>{rir(0, 0) + (pp - p.x * p.x) * m, rir(1, 1) + (pp - p.y * p.y) * m,
+                     rir(2, 2) + (pp - p.z * p.z) * m, rir(0, 1) - p.x * p.y * m,
+                     rir(0, 2) - p.x * p.z * m, rir(1, 2) - p.y * p.z * m};
+    cc[l] = p * m;
+  }
+}
+
+// cdof: dof motions about the root COM in world axes (dynamics.py:56-89)
+template <class T, class S>
+BGX_D void dof_motions(const float* __restrict__ M, const S* q, const V3T<S>* xp,
+                       const Q4T<S>* xr, V3T<S> rc, V3T<S>* ca, V3T<S>* cv) {
+  const S zero = lift<S>(0.f);
+#pragma unroll
+  for (int l = 0; l < T::L; ++l) {
+    const float* lk = link_rec<T>(M, l);
+    const int par = T::PARENT[l], da = T::DADR[l], qa = T::QADR[l];
+    V3T<S> Pp{zero, zero, zero};
+    Q4T<S> Pr = lift4<S>(qident());
+    if (T::LTYPE[l] == 0) {
+      Pp = xp[l], Pr = xr[l];
+    } else if (par >= 0) {
+      Pp = xp[par], Pr = xr[par];
+    }
+    const V3T<S> t1p = Pp + rotate(lift3<S>(ldv3(lk + LK_TPOS)), Pr);
+    const Q4T<S> jr = qmul(Pr, lift4<S>(ldq4(lk + LK_TROT)));
+    const V3T<S> jp = t1p + rotate(lift3<S>(ldv3(lk + LK_JPOS)), jr);
+    const int nd = T::LTYPE[l] == 0 ? 6 : T::LTYPE[l];
+    V3T<S> accp{zero, zero, zero};
+    Q4T<S> accr = lift4<S>(qident());
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      if (k < nd) {
+        const int d = da + k;
+        const V3 ang = dof_ang<T>(M, d), vel = dof_vel<T>(M, d);
+        V3T<S> a = lift3<S>(ang), v = lift3<S>(vel);
+        if (T::LTYPE[l] != 0) {  // stacked joints (dynamics.py:59-83)
+          a = rotate(lift3<S>(ang), accr);
+          v = rotate(lift3<S>(vel), accr) + cross(accp, a);
+          const Q4T<S> rk = qnormalize(qrot_axis(lift3<S>(ang), q[qa + k]));
+          accp = accp + rotate(vel * q[qa + k], accr);
+          accr = qmul(accr, rk);
+        }
+        a = rotate(a, jr);  // dynamics.py:86-89
+        ca[d] = a;
+        cv[d] = v - cross(rc - jp, a);
+      }
+    }
+  }
+}
+
+// inverse dynamics bias (dynamics.py:105-184): RNEA with the root at -g and
+// cdofd formed on the fly; cda, cdv are the link velocities at qd
+template <class T, class S>
+BGX_D void rnea_bias(const float* __restrict__ M, const V3T<S>* ca, const V3T<S>* cv,
+                     const V3T<S>* cda, const V3T<S>* cdv, const V3T<S>* cc,
+                     const Sym3T<S>* ci, const S* qd, S* bias) {
+  const S zero = lift<S>(0.f);
+  const V3T<S> z3{zero, zero, zero};
+  const V3T<S> grav = lift3<S>(ldv3(M + H_GRAVITY));
+  V3T<S> aa[T::L], av[T::L], fa[T::L], fv[T::L];
+#pragma unroll
+  for (int l = 0; l < T::L; ++l) {
+    const int par = T::PARENT[l], da = T::DADR[l];
+    V3T<S> a = par < 0 ? z3 : aa[par];
+    V3T<S> v = par < 0 ? -grav : av[par];
+    if (T::LTYPE[l] == 0) {
+      // free joint: cdofd = (sum of the translational cdof qd) x cdof for
+      // the rotational dofs, zero for the translational ones (:107-112)
+      V3T<S> sa = z3, sv = z3;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        sa = sa + ca[da + k] * qd[da + k];
+        sv = sv + cv[da + k] * qd[da + k];
+      }
+#pragma unroll
+      for (int k = 3; k < 6; ++k) {
+        V3T<S> oa, ov;
+        cross_motion(sa, sv, ca[da + k], cv[da + k], oa, ov);
+        a = a + oa * qd[da + k];
+        v = v + ov * qd[da + k];
+      }
+    } else {
+      V3T<S> sa = par >= 0 ? cda[par] : z3;
+      V3T<S> sv = par >= 0 ? cdv[par] : z3;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        if (k < T::LTYPE[l]) {
+          const int d = da + k;
+          if (k > 0) {
+            sa = sa + ca[d - 1] * qd[d - 1];
+            sv = sv + cv[d - 1] * qd[d - 1];
+          }
+          V3T<S> oa, ov;
+          cross_motion(sa, sv, ca[d], cv[d], oa, ov);
+          a = a + oa * qd[d];
+          v = v + ov * qd[d];
+        }
+      }
+    }
+    aa[l] = a, av[l] = v;
+    const float m = link_rec<T>(M, l)[LK_MASS];
+    V3T<S> f1a, f1v, f2a, f2v, f3a, f3v;
+    inertia_mul(cc[l], ci[l], m, a, v, f1a, f1v);
+    inertia_mul(cc[l], ci[l], m, cda[l], cdv[l], f2a, f2v);
+    cross_force(cda[l], cdv[l], f2a, f2v, f3a, f3v);
+    fa[l] = f1a + f3a, fv[l] = f1v + f3v;
+  }
+  project_forces_on<T>(ca, cv, fa, fv, bias);
+}
+
+// qf_smooth = passive - bias + tau (dynamics.py:187-236, actuator.py:23-57);
+// qf holds the bias on entry
+template <class T, class S>
+BGX_D void smooth_forces(const float* __restrict__ M, const S* q, const S* qd,
+                         const float* act, S* qf) {
+#pragma unroll
+  for (int d = 0; d < T::NV; ++d) {
+    const float* df = dof_rec<T>(M, d);
+    S p = -df[DF_DAMPING] * qd[d];
+    const int l = T::DOF_LINK[d];
+    if (T::LTYPE[l] != 0)
+      p = -q[T::QADR[l] + (d - T::DADR[l])] * df[DF_STIFFNESS] + p;
+    qf[d] = p - qf[d];
+  }
+#pragma unroll
+  for (int a = 0; a < T::NU; ++a) {
+    const float* ac = M + T::OFF_ACT + a * ACT_STRIDE;
+    const int qa = T::ACT_Q[a], da = T::ACT_D[a];
+    const float ctl = clipf(act[a], ac[AC_CTRL_LO], ac[AC_CTRL_HI]);
+    const S b = ac[AC_GEAR] * (q[qa] * ac[AC_BIAS_Q] + qd[da] * ac[AC_BIAS_QD]);
+    S f = ac[AC_GAIN] * ctl + b;
+    const float fv = val(f);
+    if (fv < ac[AC_FORCE_LO]) f = lift<S>(ac[AC_FORCE_LO]);
+    if (fv > ac[AC_FORCE_HI]) f = lift<S>(ac[AC_FORCE_HI]);
+    qf[da] += f * ac[AC_GEAR];
+  }
+}
+
+// mass.matrix (mass.py:27-83): composite rigid-body inertias (cc, ci turn into
+// subtree sums in place), then M_ij = cdof_j . (I_c(i) cdof_i) on the
+// dof-ancestor pattern, lower part, with armature on the diagonal
+template <class T, class S>
+BGX_D void crba(const float* __restrict__ M, V3T<S>* cc, Sym3T<S>* ci,
+                const V3T<S>* ca, const V3T<S>* cv, S (*Ml)[T::NV]) {
+  float km[T::L];
+#pragma unroll
+  for (int l = 0; l < T::L; ++l) km[l] = link_rec<T>(M, l)[LK_MASS];
+#pragma unroll
+  for (int l = T::L - 1; l >= 0; --l) {
+    const int par = T::PARENT[l];
+    if (par >= 0) {
+      cc[par] = cc[par] + cc[l];
+      ci[par] = symadd(ci[par], ci[l]);
+      km[par] = km[par] + km[l];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < T::NV; ++i) {
+    const int l = T::DOF_LINK[i];
+    V3T<S> fa, fv;
+    inertia_mul(cc[l], ci[l], km[l], ca[i], cv[i], fa, fv);
+    Ml[i][i] = dot(cv[i], fv) + dot(ca[i], fa) + dof_rec<T>(M, i)[DF_ARMATURE];
+#pragma unroll
+    for (int j = 0; j < i; ++j)
+      if ((T::DANC[i] >> j) & 1u) Ml[i][j] = dot(cv[j], fv) + dot(ca[j], fa);
+  }
+}
 
 // ---------------------------------------------------------------------------
 // impedance and reference acceleration (constraint.py:29-65); im is an
 // impedance record (bgx_layout.h) with the parameter-only factors
 // precomputed on the host
 
-BGX_D float impedance(const float* __restrict__ im, float pos) {
+template <class S>
+BGX_D S impedance(const float* __restrict__ im, S pos) {
   const float dmin = im[IM_DMIN], dmax = im[IM_DMAX], mid = im[IM_MID],
               power = im[IM_POWER];
-  const float x = fabsf(pos) * im[IM_INVWIDTH];
-  float xa, xb;
+  const S x = fabs_(pos) * im[IM_INVWIDTH];
+  S xa, xb;
   if (power == 2.f) {
     xa = x * x;
     xb = (1.f - x) * (1.f - x);
   } else {
-    xa = powf(x, power);
-    xb = powf(1.f - x, power);
+    xa = pow_(x, power);
+    xb = pow_(1.f - x, power);
   }
-  const float ia = im[IM_ACOEF] * xa;
-  const float ib = 1.f - im[IM_BCOEF] * xb;
-  float imp = dmin + (x < mid ? ia : ib) * (dmax - dmin);
-  imp = clipf(imp, dmin, dmax);
-  return x > 1.f ? dmax : imp;
+  const S ia = im[IM_ACOEF] * xa;
+  const S ib = 1.f - im[IM_BCOEF] * xb;
+  S imp = dmin + (val(x) < mid ? ia : ib) * (dmax - dmin);
+  // jp.clip(imp, dmin, dmax), then imp = dmax beyond the width
+  if (val(imp) < dmin) imp = lift<S>(dmin);
+  if (val(imp) > dmax) imp = lift<S>(dmax);
+  return val(x) > 1.f ? lift<S>(dmax) : imp;
 }
 
 // (1 - imp) / (imp + 1e-8): the diagonal regularizer factor
-BGX_D float imp_ratio(float imp) { return (1.f - imp) * rcp(imp + 1e-8f); }
+template <class S>
+BGX_D S imp_ratio(S imp) {
+  return (1.f - imp) * rcp(imp + 1e-8f);
+}
 
 // ---------------------------------------------------------------------------
 // contact geometry of candidate c: MJX plane-sphere / plane-capsule
 
-template <class T>
-BGX_D void candidate_center(const float* __restrict__ M, int c, const V3* xp,
-                            const Q4* xr, V3& ctr, V3& axis) {
+template <class T, class S>
+BGX_D void candidate_center(const float* __restrict__ M, int c, const V3T<S>* xp,
+                            const Q4T<S>* xr, V3T<S>& ctr, V3T<S>& axis) {
   const int l = T::CAND_LINK[c];
   const float* cd = cand_rec<T>(M, c);
   // geom world pose (contact.py:42-53)
-  const V3 gp = xp[l] + rotate(ldv3(cd + CD_GPOS), xr[l]);
+  const V3T<S> gp = xp[l] + rotate(lift3<S>(ldv3(cd + CD_GPOS)), xr[l]);
   ctr = gp;
-  axis = v3zero();
+  const S zero = lift<S>(0.f);
+  axis = V3T<S>{zero, zero, zero};
   if (T::CAND_KIND[c] == KIND_CAPSULE) {
-    const Q4 gq = qmul(xr[l], ldq4(cd + CD_GQUAT));
+    const Q4T<S> gq = qmul(xr[l], lift4<S>(ldq4(cd + CD_GQUAT)));
     axis = quat_zaxis(gq);
     ctr = gp + axis * (cd[CD_HALFLEN] * cd[CD_SIGN]);
   }
 }
 
-template <class T>
-BGX_D float candidate_dist(const float* __restrict__ M, int c, V3 ctr) {
+template <class T, class S>
+BGX_D S candidate_dist(const float* __restrict__ M, int c, V3T<S> ctr) {
   const float* cd = cand_rec<T>(M, c);
   if (T::PLANE_Z) return (ctr.z - cd[CD_PPOS + 2]) - cd[CD_RADIUS];
-  return dot(ctr - ldv3(cd + CD_PPOS), ldv3(cd + CD_PN)) - cd[CD_RADIUS];
+  return dot(ctr - lift3<S>(ldv3(cd + CD_PPOS)), lift3<S>(ldv3(cd + CD_PN))) -
+         cd[CD_RADIUS];
+}
+
+// contact point and frame of an active candidate (MJX plane collisions):
+// pos = center - n (radius + dist / 2); sphere: the host-computed frame;
+// capsule: t1 along the capsule axis projected on the plane, or a fixed axis
+// when the capsule is near-normal (fallback < 0: decide here; 0 / 1: use the
+// given decision); t2 = n x t1.  Returns whether the fallback was taken.
+template <class T, class S>
+BGX_D bool contact_frame(const float* __restrict__ M, int c, V3T<S> ctr, V3T<S> axis,
+                         S dist, int fallback, V3T<S>& pos, V3T<S>& t1, V3T<S>& t2) {
+  const float* cd = cand_rec<T>(M, c);
+  const V3 nf = T::PLANE_Z ? v3(0.f, 0.f, 1.f) : ldv3(cd + CD_PN);
+  const V3T<S> n = lift3<S>(nf);
+  pos = ctr - n * (cd[CD_RADIUS] + 0.5f * dist);
+  t1 = lift3<S>(ldv3(cd + CD_T1)), t2 = lift3<S>(ldv3(cd + CD_T2));
+  bool fb = false;
+  if (T::CAND_KIND[c] == KIND_CAPSULE) {
+    S bn;
+    const V3T<S> b = vnormalize(axis - n * dot(n, axis), &bn);
+    const V3 alt = (-0.5f < nf.y && nf.y < 0.5f) ? v3(0.f, 1.f, 0.f) : v3(0.f, 0.f, 1.f);
+    fb = fallback < 0 ? (val(bn) < 0.5f) : (fallback != 0);
+    t1 = fb ? lift3<S>(alt) : b;
+    t2 = cross(n, t1);
+  }
+  return fb;
+}
+
+// ---------------------------------------------------------------------------
+// position update of the semi-implicit Euler step (integrator.py:26-44)
+
+// free-joint rotation: rot (x) quat_rot_axis(w / (|w| + 1e-8), dt (|w| + 1e-8)),
+// renormalized
+template <class S>
+BGX_D Q4T<S> integrate_rot(Q4T<S> rot, V3T<S> ang, float dt) {
+  const S an = fsqrt(dot(ang, ang)) + 1e-8f;
+  const S ian = rcp(an);
+  const Q4T<S> qr = qrot_axis(V3T<S>{ang.x * ian, ang.y * ian, ang.z * ian}, dt * an);
+  const Q4T<S> r = qmul(rot, qr);
+  const S rn = rsq(r.w * r.w + r.x * r.x + r.y * r.y + r.z * r.z);
+  return Q4T<S>{r.w * rn, r.x * rn, r.y * rn, r.z * rn};
+}
+
+// q += qd dt for axis joints and the free-joint position; rotation as above
+template <class T>
+BGX_D void integrate_q(float* q, const float* qd, float dt) {
+#pragma unroll
+  for (int l = 0; l < T::L; ++l) {
+    const int qa = T::QADR[l], da = T::DADR[l];
+    if (T::LTYPE[l] == 0) {
+      const Q4 rot = integrate_rot(Q4{q[qa + 3], q[qa + 4], q[qa + 5], q[qa + 6]},
+                                   v3(qd[da + 3], qd[da + 4], qd[da + 5]), dt);
+      q[qa + 3] = rot.w, q[qa + 4] = rot.x, q[qa + 5] = rot.y, q[qa + 6] = rot.z;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) q[qa + k] = q[qa + k] + qd[da + k] * dt;
+    } else {
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        if (k < T::LTYPE[l]) q[qa + k] = q[qa + k] + qd[da + k] * dt;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -345,9 +687,9 @@ BGX_D void slot_dots(const SlotGeo<false>& g, V3 vp, float& e0, float& e1, float
 }
 
 // per-row solver vectors of the slots: x, y (FISTA iterates), g (gradient),
-// r (residual), b.  Fast path: LDS, element e of this lane at lds[e * WAVE];
-// overflow path: a private array.
-constexpr int RV_X = 0, RV_Y = 1, RV_G = 2, RV_R = 3, RV_B = 4, RV_N = 5;
+// r (residual).  Fast path: LDS, element e of this lane at lds[e * WAVE];
+// overflow path: a private array.  (b stays in registers.)
+constexpr int RV_X = 0, RV_Y = 1, RV_G = 2, RV_R = 3, RV_N = 4;
 template <int KS>
 struct LdsRows {
   float* base;
@@ -367,7 +709,7 @@ struct PrivRows {
 // link scratch (6 floats per link) followed by space for 54 more floats
 
 template <class T, int KS, class Rows>
-BGX_D void substep_impl(const float* __restrict__ M, float* q, float* qd,
+BGX_D bool substep_impl(const float* __restrict__ M, float* q, float* qd,
                         const float* act, const V3* xp, const Q4* xr, V3 rc0,
                         float* lds, Rows& rows, float* dbg) {
   constexpr int NV = T::NV, L = T::L;
@@ -390,18 +732,9 @@ BGX_D void substep_impl(const float* __restrict__ M, float* q, float* qd,
     if (dist < 0.f) {
       const int l = T::CAND_LINK[c];
       const float* cd = cand_rec<T>(M, c);
+      V3 pos, t1, t2;
+      contact_frame<T>(M, c, ctr, axis, dist, -1, pos, t1, t2);
       const V3 n = PZ ? v3(0.f, 0.f, 1.f) : ldv3(cd + CD_PN);
-      const V3 pos = ctr - n * (cd[CD_RADIUS] + 0.5f * dist);
-      V3 t1 = ldv3(cd + CD_T1), t2 = ldv3(cd + CD_T2);
-      if (T::CAND_KIND[c] == KIND_CAPSULE) {
-        // frame aligned with the capsule axis, fallback when near-normal
-        float bn;
-        const V3 b = vnormalize(axis - n * dot(n, axis), &bn);
-        const V3 alt = (-0.5f < n.y && n.y < 0.5f) ? v3(0.f, 1.f, 0.f)
-                                                    : v3(0.f, 0.f, 1.f);
-        t1 = bn < 0.5f ? alt : b;
-        t2 = cross(n, t1);
-      }
       const float mu = cd[CD_MU];
       SlotGeo<PZ> g;
       slot_set(g, l, pos - t.rc[l], n, t1 * mu, t2 * mu);
@@ -424,72 +757,15 @@ BGX_D void substep_impl(const float* __restrict__ M, float* q, float* qd,
       n_c += 1;
     }
   }
-  if (n_c > KS) n_c = KS;  // callers choose KS >= the active count
+  // more active contacts than slots: leave the state untouched and let the
+  // caller take the path with a slot per candidate
+  if (n_c > KS) return false;
 
   // ---- dynamics.transform_com (dynamics.py:27-89): cinr, cdof
   V3 cc[L];
   Sym3 ci[L];
-#pragma unroll
-  for (int l = 0; l < L; ++l) {
-    const float* lk = link_rec<T>(M, l);
-    const V3 xip = xp[l] + rotate(ldv3(lk + LK_IPOS), xr[l]);
-    const Q4 xir = qmul(xr[l], ldq4(lk + LK_IROT));
-    const float m = lk[LK_MASS];
-    const V3 p = xip - t.rc[l];
-    float R[3][3];
-    quat_to_3x3(xir, R);
-    const Sym3 I0 = ldsym(lk + LK_I);
-    float RI[3][3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      const V3 c = symmul(I0, v3(R[i][0], R[i][1], R[i][2]));
-      RI[i][0] = c.x, RI[i][1] = c.y, RI[i][2] = c.z;
-    }
-    auto rir = [&](int i, int j) {
-      return RI[i][0] * R[j][0] + RI[i][1] * R[j][1] + RI[i][2] * R[j][2];
-    };
-    const float pp = dot(p, p);
-    ci[l] = Sym3{rir(0, 0) + (pp - p.x * p.x) * m, rir(1, 1) + (pp - p.y * p.y) * m,
-                 rir(2, 2) + (pp - p.z * p.z) * m, rir(0, 1) - p.x * p.y * m,
-                 rir(0, 2) - p.x * p.z * m, rir(1, 2) - p.y * p.z * m};
-    cc[l] = p * m;
-  }
-#pragma unroll
-  for (int l = 0; l < L; ++l) {
-    const float* lk = link_rec<T>(M, l);
-    const int par = T::PARENT[l], da = T::DADR[l], qa = T::QADR[l];
-    V3 Pp = v3zero();
-    Q4 Pr = qident();
-    if (T::LTYPE[l] == 0) {
-      Pp = xp[l], Pr = xr[l];
-    } else if (par >= 0) {
-      Pp = xp[par], Pr = xr[par];
-    }
-    const V3 t1p = Pp + rotate(ldv3(lk + LK_TPOS), Pr);
-    const Q4 jr = qmul(Pr, ldq4(lk + LK_TROT));
-    const V3 jp = t1p + rotate(ldv3(lk + LK_JPOS), jr);
-    const int nd = T::LTYPE[l] == 0 ? 6 : T::LTYPE[l];
-    V3 accp = v3zero();
-    Q4 accr = qident();
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      if (k < nd) {
-        const int d = da + k;
-        const V3 ang = dof_ang<T>(M, d), vel = dof_vel<T>(M, d);
-        V3 a = ang, v = vel;
-        if (T::LTYPE[l] != 0) {  // stacked joints (dynamics.py:59-83)
-          a = rotate(ang, accr);
-          v = rotate(vel, accr) + cross(accp, a);
-          const Q4 rk = qnormalize(qrot_axis(ang, q[qa + k]));
-          accp = accp + rotate(vel * q[qa + k], accr);
-          accr = qmul(accr, rk);
-        }
-        a = rotate(a, jr);  // dynamics.py:86-89
-        t.ca[d] = a;
-        t.cv[d] = v - cross(t.rc[l] - jp, a);
-      }
-    }
-  }
+  com_inertia<T>(M, xp, xr, t.rc[0], cc, ci);
+  dof_motions<T>(M, q, xp, xr, t.rc[0], t.ca, t.cv);
 
   // ---- cd: link velocities at qd (dynamics.py:90-103), also kept in lds2
   V3 cda[L], cdv[L];
@@ -507,80 +783,11 @@ BGX_D void substep_impl(const float* __restrict__ M, float* q, float* qd,
       f[3 * WAVE] = cdv[l].x, f[4 * WAVE] = cdv[l].y, f[5 * WAVE] = cdv[l].z;
     }
 
-  // ---- dynamics.inverse: RNEA bias (dynamics.py:105-184), cdofd on the fly
+  // ---- dynamics.inverse: RNEA bias (dynamics.py:105-184), cdofd on the fly;
+  // then passive forces and actuators: qf_smooth = passive - bias + tau
   float qf[NV];
-  {
-    const V3 grav = ldv3(M + H_GRAVITY);
-    V3 aa[L], av[L], fa[L], fv[L];
-#pragma unroll
-    for (int l = 0; l < L; ++l) {
-      const int par = T::PARENT[l], da = T::DADR[l];
-      V3 a = par < 0 ? v3zero() : aa[par];
-      V3 v = par < 0 ? -grav : av[par];
-      if (T::LTYPE[l] == 0) {
-        // free joint: cdofd = (sum of the translational cdof qd) x cdof for
-        // the rotational dofs, zero for the translational ones (:107-112)
-        V3 sa = v3zero(), sv = v3zero();
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          sa = sa + t.ca[da + k] * qd[da + k];
-          sv = sv + t.cv[da + k] * qd[da + k];
-        }
-#pragma unroll
-        for (int k = 3; k < 6; ++k) {
-          V3 oa, ov;
-          cross_motion(sa, sv, t.ca[da + k], t.cv[da + k], oa, ov);
-          a = a + oa * qd[da + k];
-          v = v + ov * qd[da + k];
-        }
-      } else {
-        V3 sa = par >= 0 ? cda[par] : v3zero();
-        V3 sv = par >= 0 ? cdv[par] : v3zero();
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          if (k < T::LTYPE[l]) {
-            const int d = da + k;
-            if (k > 0) {
-              sa = sa + t.ca[d - 1] * qd[d - 1];
-              sv = sv + t.cv[d - 1] * qd[d - 1];
-            }
-            V3 oa, ov;
-            cross_motion(sa, sv, t.ca[d], t.cv[d], oa, ov);
-            a = a + oa * qd[d];
-            v = v + ov * qd[d];
-          }
-        }
-      }
-      aa[l] = a, av[l] = v;
-      const float m = link_rec<T>(M, l)[LK_MASS];
-      V3 f1a, f1v, f2a, f2v, f3a, f3v;
-      inertia_mul(cc[l], ci[l], m, a, v, f1a, f1v);
-      inertia_mul(cc[l], ci[l], m, cda[l], cdv[l], f2a, f2v);
-      cross_force(cda[l], cdv[l], f2a, f2v, f3a, f3v);
-      fa[l] = f1a + f3a, fv[l] = f1v + f3v;
-    }
-    project_forces<T>(t, fa, fv, qf);
-  }
-  // passive forces, actuators: qf_smooth = passive - bias + tau
-#pragma unroll
-  for (int d = 0; d < NV; ++d) {
-    const float* df = dof_rec<T>(M, d);
-    float p = -df[DF_DAMPING] * qd[d];
-    const int l = T::DOF_LINK[d];
-    if (T::LTYPE[l] != 0)
-      p = -q[T::QADR[l] + (d - T::DADR[l])] * df[DF_STIFFNESS] + p;
-    qf[d] = p - qf[d];
-  }
-#pragma unroll
-  for (int a = 0; a < T::NU; ++a) {
-    const float* ac = M + T::OFF_ACT + a * ACT_STRIDE;
-    const int qa = T::ACT_Q[a], da = T::ACT_D[a];
-    const float ctl = clipf(act[a], ac[AC_CTRL_LO], ac[AC_CTRL_HI]);
-    const float b = ac[AC_GEAR] * (q[qa] * ac[AC_BIAS_Q] + qd[da] * ac[AC_BIAS_QD]);
-    float f = ac[AC_GAIN] * ctl + b;
-    f = clipf(f, ac[AC_FORCE_LO], ac[AC_FORCE_HI]);
-    qf[da] += f * ac[AC_GEAR];
-  }
+  rnea_bias<T>(M, t.ca, t.cv, cda, cdv, cc, ci, qd, qf);
+  smooth_forces<T>(M, q, qd, act, qf);
   if (dbg) {
 #pragma unroll
     for (int d = 0; d < NV; ++d) dbg[d] = qf[d];
@@ -588,49 +795,8 @@ BGX_D void substep_impl(const float* __restrict__ M, float* q, float* qd,
 
   // ---- mass.matrix: CRBA on the dof-ancestor pattern (mass.py:27-83), then
   // the LTL factorization, leaves first
-  {
-    float km[L];
-#pragma unroll
-    for (int l = 0; l < L; ++l) km[l] = link_rec<T>(M, l)[LK_MASS];
-#pragma unroll
-    for (int l = L - 1; l >= 0; --l) {
-      const int par = T::PARENT[l];
-      if (par >= 0) {
-        cc[par] = cc[par] + cc[l];
-        ci[par] = symadd(ci[par], ci[l]);
-        km[par] = km[par] + km[l];
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-      const int l = T::DOF_LINK[i];
-      V3 fa, fv;
-      inertia_mul(cc[l], ci[l], km[l], t.ca[i], t.cv[i], fa, fv);
-      t.Lm[i][i] = dot(t.cv[i], fv) + dot(t.ca[i], fa) + dof_rec<T>(M, i)[DF_ARMATURE];
-#pragma unroll
-      for (int j = 0; j < i; ++j)
-        if ((T::DANC[i] >> j) & 1u) t.Lm[i][j] = dot(t.cv[j], fv) + dot(t.ca[j], fa);
-    }
-  }
-#pragma unroll
-  for (int k = NV - 1; k >= 0; --k) {
-    const float lkk = fsqrt(t.Lm[k][k]);
-    t.Lm[k][k] = lkk;
-    t.inv[k] = rcp(lkk);
-#pragma unroll
-    for (int j = 0; j < k; ++j)
-      if ((T::DANC[k] >> j) & 1u) t.Lm[k][j] *= t.inv[k];
-#pragma unroll
-    for (int i = 0; i < k; ++i) {
-      if ((T::DANC[k] >> i) & 1u) {
-        const float a = t.Lm[k][i];
-        t.Lm[i][i] -= a * a;
-#pragma unroll
-        for (int j = 0; j < i; ++j)
-          if ((T::DANC[i] >> j) & 1u) t.Lm[i][j] -= a * t.Lm[k][j];
-      }
-    }
-  }
+  crba<T>(M, cc, ci, t.ca, t.cv, t.Lm);
+  ltl_factor<T>(t);
 
   // ---- unconstrained acceleration u = M^-1 qf_smooth
   float u[NV];
@@ -676,10 +842,7 @@ BGX_D void substep_impl(const float* __restrict__ M, float* q, float* qd,
   for (int s = 0; s < KS; ++s)
     if (s < n_c)
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        rows.at(RV_B, s, k) = sb[s][k];
-        rows.at(RV_X, s, k) = rows.at(RV_Y, s, k) = 0.f;
-      }
+      for (int k = 0; k < 4; ++k) rows.at(RV_X, s, k) = rows.at(RV_Y, s, k) = 0.f;
 
   // ---- limit rows (constraint.py:101-132), one per non-free dof
   float lside[NL], ldiag[NL], lb[NL];
@@ -796,7 +959,7 @@ BGX_D void substep_impl(const float* __restrict__ M, float* q, float* qd,
     apply_a([&](int s, int k) { return rows.at(RV_Y, s, k); },
             [&](int i) { return ly[i]; },
             [&](int s, int k, float o) {
-              const float r = o + rows.at(RV_B, s, k);
+              const float r = o + sb[s][k];
               rows.at(RV_R, s, k) = r;
               fy += 0.5f * r * r;
             },
@@ -820,7 +983,7 @@ BGX_D void substep_impl(const float* __restrict__ M, float* q, float* qd,
               },
               [&](int i) { return fmaxf(ly[i] - e_ * lg[i], 0.f); },
               [&](int s, int k, float o) {
-                const float r = o + rows.at(RV_B, s, k);
+                const float r = o + sb[s][k];
                 rows.at(RV_R, s, k) = r;
                 fx += 0.5f * r * r;
                 const float yv = rows.at(RV_Y, s, k), gv = rows.at(RV_G, s, k);
@@ -901,71 +1064,41 @@ BGX_D void substep_impl(const float* __restrict__ M, float* q, float* qd,
   const float dt = M[H_DT];
 #pragma unroll
   for (int d = 0; d < NV; ++d) qd[d] = qd[d] + qdd[d] * dt;
-#pragma unroll
-  for (int l = 0; l < L; ++l) {
-    const int qa = T::QADR[l], da = T::DADR[l];
-    if (T::LTYPE[l] == 0) {
-      const V3 ang = v3(qd[da + 3], qd[da + 4], qd[da + 5]);
-      const float an = fsqrt(dot(ang, ang)) + 1e-8f;
-      const float ian = rcp(an);
-      const Q4 qr = qrot_axis(v3(ang.x * ian, ang.y * ian, ang.z * ian), dt * an);
-      Q4 rot = qmul(Q4{q[qa + 3], q[qa + 4], q[qa + 5], q[qa + 6]}, qr);
-      const float rn = rsq(rot.w * rot.w + rot.x * rot.x + rot.y * rot.y + rot.z * rot.z);
-      q[qa + 3] = rot.w * rn;
-      q[qa + 4] = rot.x * rn;
-      q[qa + 5] = rot.y * rn;
-      q[qa + 6] = rot.z * rn;
-#pragma unroll
-      for (int k = 0; k < 3; ++k) q[qa + k] = q[qa + k] + qd[da + k] * dt;
-    } else {
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
-        if (k < T::LTYPE[l]) q[qa + k] = q[qa + k] + qd[da + k] * dt;
-    }
-  }
+  integrate_q<T>(q, qd, dt);
+  return true;
 }
 
-// one substep: kinematics at (q, qd), then the fast path (active contacts in
-// KC register/LDS slots) or, when more candidates touch, the overflow path
-// (all candidates, private arrays)
+// kinematics, root COM, then substep_impl with KS slots
+template <class T, int KS, class Rows>
+BGX_D bool substep_ks(const float* __restrict__ M, float* q, float* qd,
+                      const float* act, float* lds, Rows& rows, float* dbg) {
+  static_assert(T::L > 0, "empty topology");
+  static_assert(T::ROOT[T::L - 1] == 0, "one kinematic tree per topology");
+  V3 xp[T::L];
+  Q4 xr[T::L];
+  forward_kinematics<T, false, float>(M, q, qd, xp, xr, nullptr, nullptr);
+  const V3 rc = root_com<T>(M, xp, xr);  // dynamics.py:37-43
+  return substep_impl<T, KS>(M, q, qd, act, xp, xr, rc, lds, rows, dbg);
+}
+
+// an opaque copy of a uniform pointer: loads through it are not merged with
+// loads through the original
+BGX_D const float* launder(const float* p) {
+  asm volatile("" : "+s"(p));
+  return p;
+}
+
+// one substep: the fast path keeps the active contacts in KC slots (rows in
+// LDS); when more candidates touch the floor it returns before changing the
+// state and the overflow path redoes the substep with a slot per candidate
 template <class T>
 BGX_D int substep(const float* __restrict__ M, float* q, float* qd,
                   const float* act, float* lds, float* dbg) {
-  static_assert(T::L > 0, "empty topology");
-  V3 xp[T::L];
-  Q4 xr[T::L];
-  forward_kinematics<T, false>(M, q, qd, xp, xr, nullptr, nullptr);
-  // root COM of the tree (dynamics.py:37-43); one root per topology
-  static_assert(T::ROOT[T::L - 1] == 0, "one kinematic tree per topology");
-  V3 rc;
-  {
-    V3 sm = v3zero();
-    float ms = 0.f;
-#pragma unroll
-    for (int l = 0; l < T::L; ++l) {
-      const float* lk = link_rec<T>(M, l);
-      const float m = lk[LK_MASS];
-      sm = sm + (xp[l] + rotate(ldv3(lk + LK_IPOS), xr[l])) * m;
-      ms = ms + m;
-    }
-    const float ims = rcp(ms);
-    rc = v3(sm.x * ims, sm.y * ims, sm.z * ims);
-  }
-  int n_act = 0;
-#pragma unroll
-  for (int c = 0; c < T::NCAND; ++c) {
-    V3 ctr, axis;
-    candidate_center<T>(M, c, xp, xr, ctr, axis);
-    n_act += candidate_dist<T>(M, c, ctr) < 0.f ? 1 : 0;
-  }
-  if (n_act <= T::KC) {
-    LdsRows<T::KC> rows{lds + T::L * 6 * WAVE};
-    substep_impl<T, T::KC>(M, q, qd, act, xp, xr, rc, lds, rows, dbg);
-    return 0;
-  }
+  LdsRows<T::KC> rows{lds + T::L * 6 * WAVE};
+  if (substep_ks<T, T::KC>(M, q, qd, act, lds, rows, dbg)) return 0;
   constexpr int KB = T::NCAND > T::KC ? T::NCAND : T::KC;
-  PrivRows<KB> rows;
-  substep_impl<T, KB>(M, q, qd, act, xp, xr, rc, lds, rows, dbg);
+  PrivRows<KB> prow;
+  substep_ks<T, KB>(launder(M), q, qd, act, lds, prow, dbg);
   return ST_SLOW_PATH;
 }
 

@@ -213,3 +213,74 @@ def probe(sys: S.System, state: State, act, n_frames: int = 1) -> Dict[str, torc
   return {"q": q, "qd": qd, "qf_smooth": out[:, :nv], "qdd": out[:, nv:2 * nv],
           "n_contacts": out[:, 2 * nv], "iters": out[:, 2 * nv + 1],
           "trials": out[:, 2 * nv + 2]}
+
+
+class _RolloutFn(torch.autograd.Function):
+  """Differentiable rollout: forward = bgx_rollout with the trajectory,
+  backward = bgx_rollout_adjoint (segments of ckpt_every env steps are
+  recomputed from their checkpoints)."""
+
+  @staticmethod
+  def forward(ctx, q0, qd0, acts, m, n_frames, ckpt_every):
+    n, T = q0.shape[0], acts.shape[0]
+    q = q0.detach().contiguous().clone()
+    qd = qd0.detach().contiguous().clone()
+    a = acts.detach().contiguous()
+    qt = torch.empty(T, n, m.nq, device=m.device)
+    qdt = torch.empty(T, n, m.nv, device=m.device)
+    status = torch.zeros(n, dtype=torch.int32, device=m.device)
+    lib = _lib.load()
+    _lib.check(lib.bgx_rollout(m.handle, n, T, int(n_frames), _lib.ptr(q), _lib.ptr(qd),
+                               _lib.ptr(a) if m.nu else None, n * m.nu, _lib.ptr(qt),
+                               _lib.ptr(qdt), None, None, None, None, _lib.ptr(status),
+                               _stream()))
+    idx = list(range(ckpt_every - 1, T - 1, ckpt_every))
+    ckq = torch.cat([q0.detach()[None].contiguous(), qt[idx]], 0).contiguous()
+    ckqd = torch.cat([qd0.detach()[None].contiguous(), qdt[idx]], 0).contiguous()
+    ctx.save_for_backward(ckq, ckqd, a)
+    ctx.m, ctx.n_frames, ctx.ckpt_every = m, int(n_frames), int(ckpt_every)
+    ctx.mark_non_differentiable(status)
+    return qt, qdt, status
+
+  @staticmethod
+  def backward(ctx, gqt, gqdt, _gstatus):
+    ckq, ckqd, a = ctx.saved_tensors
+    m = ctx.m
+    T, n = a.shape[0], ckq.shape[1]
+    gqt = None if gqt is None else gqt.contiguous().float()
+    gqdt = None if gqdt is None else gqdt.contiguous().float()
+    lib = _lib.load()
+    nbytes = lib.bgx_adjoint_workspace_bytes(m.handle, n, ctx.ckpt_every, ctx.n_frames)
+    if nbytes < 0:
+      raise _lib.BgxError("bad adjoint workspace arguments")
+    ws = torch.empty(nbytes // 4 + 1, device=m.device)
+    gact = torch.empty(T, n, m.nu, device=m.device)
+    gq0 = torch.empty(n, m.nq, device=m.device)
+    gqd0 = torch.empty(n, m.nv, device=m.device)
+    status = torch.zeros(n, dtype=torch.int32, device=m.device)
+    _lib.check(lib.bgx_rollout_adjoint(
+        m.handle, n, T, ctx.n_frames, ctx.ckpt_every, _lib.ptr(ckq), _lib.ptr(ckqd),
+        _lib.ptr(a) if m.nu else None, n * m.nu, _lib.ptr(gqt), _lib.ptr(gqdt),
+        _lib.ptr(gact) if m.nu else None, _lib.ptr(gq0), _lib.ptr(gqd0), _lib.ptr(ws),
+        nbytes, _lib.ptr(status), _stream()))
+    return gq0, gqd0, gact, None, None, None
+
+
+def rollout_grad(sys: S.System, q0: torch.Tensor, qd0: torch.Tensor, acts: torch.Tensor,
+                 n_frames: int, ckpt_every: int = 16):
+  """Differentiable rollout of ``acts.shape[0]`` env steps.
+
+  Returns (q_traj (T, N, q_size), qd_traj (T, N, qd_size), status (N)); the
+  trajectories carry gradients to q0, qd0 and acts through the hand-written
+  adjoint (the reference obtains them with jax.grad through its scan of
+  pipeline.step, brax/training/agents/apg/train.py:188-198).  The backward
+  pass recomputes segments of ``ckpt_every`` env steps from checkpoints.
+  """
+  m = device_model(sys, q0.device)
+  if q0.dim() != 2 or q0.shape[1] != m.nq or qd0.shape != (q0.shape[0], m.nv):
+    raise ValueError("q0 must be (N, q_size) and qd0 (N, qd_size)")
+  if acts.dim() != 3 or acts.shape[1:] != (q0.shape[0], m.nu):
+    raise ValueError(f"acts must have shape (T, {q0.shape[0]}, {m.nu})")
+  if ckpt_every < 1 or n_frames < 1:
+    raise ValueError("ckpt_every and n_frames must be positive")
+  return _RolloutFn.apply(q0.float(), qd0.float(), acts.float(), m, n_frames, ckpt_every)

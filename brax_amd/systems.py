@@ -846,6 +846,123 @@ def quadruped(seed: int = 0) -> System:
   return _quadruped_builder(seed, knee_pitch_deg=60.0).build()
 
 
+def humanoid(seed: int = 0) -> System:
+  """Synthetic humanoid-like tree (16384-env workload).
+
+  13 capsule bodies: pelvis (free joint), torso (1-DoF waist), head, two
+  thighs (3-DoF hips as hinge triples x, z, y), two shins (1-DoF knees), two
+  feet (two heel-to-toe capsules each), two upper arms (3-DoF shoulders as
+  hinge triples x, y, z) and two forearms (1-DoF elbows) each carrying a hand
+  capsule.  The head and feet have no joint and are fused into their parents
+  (brax/io/mjcf.py:76-98), leaving 10 links; qd size 23, q size 24.  Body
+  masses U(1, 8) kg (split over a body's capsules by volume).  Contacts: the
+  4 foot and 2 hand capsules and a torso sphere against the plane (13 MJX
+  contacts); every hinge is limited (17 limit rows).  Options and joint
+  parameters follow brax/envs/assets/humanoid.xml:2-9,23-24,46-108:
+  timestep 0.003, 8 solver iterations, 15 line-search steps, 20 Newton-Schulz
+  iterations, motors with ctrl range [-0.4, 0.4] and gears 100/300/200/25.
+  """
+  rng = np.random.default_rng(seed)
+  masses = iter(rng.uniform(1.0, 8.0, size=13))
+  b = ModelBuilder("humanoid", timestep=0.003, iterations=8, solver_maxls=15,
+                   matrix_inv_iterations=20)
+  b.add_geom(0, GEOM_PLANE, (20, 20, 0.125), contype=0, conaffinity=1,
+             friction=(1.0, 0.1, 0.1), name="floor")
+  touch = dict(contype=1, conaffinity=0)
+  none = dict(contype=0, conaffinity=0)
+
+  def capsules(body, specs, **kw):
+    """Adds capsules (fromto, radius, contact kw) sharing one body mass."""
+    m = next(masses)
+    vols = []
+    for ft, r, _ in specs:
+      half = 0.5 * np.linalg.norm(np.subtract(ft[3:], ft[:3]))
+      vols.append(_geom_volume(GEOM_CAPSULE, np.array([r, half, 0.0])))
+    for (ft, r, ckw), v in zip(specs, vols):
+      b.add_geom(body, GEOM_CAPSULE, (r,), fromto=ft, mass=m * v / sum(vols),
+                 **ckw, **kw)
+
+  def hinge(body, name, axis, deg, armature, damping, stiffness):
+    b.add_hinge(body, name, axis, range_rad=np.deg2rad(deg),
+                armature=armature, damping=damping, stiffness=stiffness)
+
+  pelvis = b.add_body("pelvis", 0, pos=(0, 0, 0.885))
+  b.add_free_joint(pelvis)
+  capsules(pelvis, [((0, -0.07, 0, 0, 0.07, 0), 0.09, none)])
+  torso = b.add_body("torso", pelvis, pos=(0, 0, 0.1))
+  hinge(torso, "abdomen_y", (0, 1, 0), (-45, 30), 0.02, 5.0, 10.0)
+  capsules(torso, [((0, 0, 0, 0, 0, 0.3), 0.07, none)])
+  b.add_geom(torso, GEOM_SPHERE, (0.11,), pos=(0, 0, 0.3), mass=0.0,
+             name="chest", **touch)
+  head = b.add_body("head", torso, pos=(0, 0, 0.45))
+  capsules(head, [((0, 0, 0, 0, 0, 0.1), 0.08, none)])
+  for side, s in (("right", -1.0), ("left", 1.0)):
+    thigh = b.add_body(f"{side}_thigh", pelvis, pos=(0, s * 0.1, -0.04))
+    hinge(thigh, f"{side}_hip_x", (s, 0, 0), (-25, 5), 0.01, 5.0, 10.0)
+    hinge(thigh, f"{side}_hip_z", (0, 0, s), (-60, 35), 0.01, 5.0, 10.0)
+    hinge(thigh, f"{side}_hip_y", (0, 1, 0), (-110, 20), 0.008, 5.0, 20.0)
+    capsules(thigh, [((0, 0, 0, 0, 0, -0.34), 0.06, none)])
+    shin = b.add_body(f"{side}_shin", thigh, pos=(0, 0, -0.4))
+    hinge(shin, f"{side}_knee", (0, -1, 0), (-150, 0), 0.006, 1.0, 0.0)
+    capsules(shin, [((0, 0, 0, 0, 0, -0.3), 0.05, none)])
+    foot = b.add_body(f"{side}_foot", shin, pos=(0, 0, -0.4))
+    capsules(foot, [((-0.05, -0.03, 0, 0.12, -0.03, 0), 0.03, touch),
+                    ((-0.05, 0.03, 0, 0.12, 0.03, 0), 0.03, touch)])
+  for side, s in (("right", -1.0), ("left", 1.0)):
+    uarm = b.add_body(f"{side}_upper_arm", torso, pos=(0, s * 0.17, 0.25))
+    hinge(uarm, f"{side}_shoulder_x", (s, 0, 0), (-85, 60), 0.0068, 1.0, 1.0)
+    hinge(uarm, f"{side}_shoulder_y", (0, 1, 0), (-85, 60), 0.0051, 1.0, 1.0)
+    hinge(uarm, f"{side}_shoulder_z", (0, 0, s), (-60, 60), 0.005, 1.0, 1.0)
+    capsules(uarm, [((0, 0, 0, 0, 0, -0.26), 0.04, none)])
+    farm = b.add_body(f"{side}_forearm", uarm, pos=(0, 0, -0.28))
+    hinge(farm, f"{side}_elbow", (0, 1, 0), (-90, 0), 0.0028, 1.0, 0.0)
+    capsules(farm, [((0, 0, 0, 0, 0, -0.22), 0.035, none),
+                    ((0, 0, -0.25, 0, 0, -0.33), 0.03, touch)])
+  b.bodies = _fuse_jointless(b)
+  gears = {"abdomen_y": 100.0}
+  for side in ("right", "left"):
+    gears.update({f"{side}_hip_x": 100.0, f"{side}_hip_z": 100.0,
+                  f"{side}_hip_y": 300.0, f"{side}_knee": 200.0,
+                  f"{side}_shoulder_x": 25.0, f"{side}_shoulder_y": 25.0,
+                  f"{side}_shoulder_z": 25.0, f"{side}_elbow": 25.0})
+  for j, g in gears.items():
+    b.add_motor(j, g, (-0.4, 0.4))
+  return b.build()
+
+
+def _fuse_jointless(b: "ModelBuilder") -> List[_Body]:
+  """Fuses bodies without joints into their parents (brax/io/mjcf.py:76-98).
+
+  Geoms of a fused body move to the parent with the body's offset applied
+  (body quaternions of fused bodies must be identity); children are re-parented
+  and body ids are renumbered.  Returns the new body list and rewrites geom
+  body ids in place.
+  """
+  keep, newid = [], {0: 0}
+  offset = {0: np.zeros(3)}
+  target = {0: 0}
+  for i, body in enumerate(b.bodies, start=1):
+    p = body.parent
+    if body.joints:
+      keep.append(body)
+      newid[i] = len(keep)
+      target[i] = i
+      offset[i] = np.zeros(3)
+    else:
+      if not np.allclose(body.quat, (1, 0, 0, 0)):
+        raise NotImplementedError("fused bodies must have identity rotation")
+      target[i] = target[p]
+      offset[i] = offset[p] + body.pos
+  for body in keep:
+    body.pos = body.pos + offset[body.parent]
+    body.parent = newid[target[body.parent]]
+  for g in b.geoms:
+    if g.body:
+      g.pos = g.pos + offset[g.body]
+      g.body = newid[target[g.body]]
+  return keep
+
+
 def ant() -> System:
   """The ant of brax/envs/assets/ant.xml after body fusing (mjcf.py:76-98).
 

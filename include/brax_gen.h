@@ -103,6 +103,38 @@ int bgx_step_probe(const bgx_model* model, int32_t n_envs, int32_t n_frames,
                    float* q, float* qd, const float* act, float* probe,
                    void* stream);
 
+/* Workspace bytes bgx_rollout_adjoint needs for n_envs, a checkpoint every
+ * ckpt_every env steps and n_frames substeps per env step (-1 on bad
+ * arguments). */
+int64_t bgx_adjoint_workspace_bytes(const bgx_model* model, int32_t n_envs,
+                                    int32_t ckpt_every, int32_t n_frames);
+
+/* Reverse-mode gradient of a rollout (bgx_rollout with n_steps env steps of
+ * n_frames substeps), replacing jax.grad through the reference's scan of
+ * pipeline.step (brax/envs/base.py:128-137, differentiated as in
+ * brax/training/agents/apg/train.py:188-198; the solver is differentiated
+ * unrolled with its decisions fixed, as jaxopt implicit_diff=False does at
+ * brax/generalized/constraint.py:229-238).
+ *   ckpt_q (n_ckpt, N, NQ), ckpt_qd (n_ckpt, N, NV): the state before env
+ *     steps 0, ckpt_every, 2 ckpt_every, ... (n_ckpt = ceil(n_steps /
+ *     ckpt_every)); each segment is recomputed from its checkpoint.
+ *   act: as bgx_rollout (NULL = zero action).
+ *   gq_traj (n_steps, N, NQ), gqd_traj (n_steps, N, NV): cotangents of the
+ *     state after each env step (NULL = zero).
+ *   Outputs: gact (n_steps, N, NU), gq0 (N, NQ), gqd0 (N, NV): cotangents of
+ *     the actions and of the initial state.  status (N, may be NULL)
+ *     receives BGX_STATUS_* bits of the recomputed substeps.
+ * workspace: device memory of bgx_adjoint_workspace_bytes() bytes.  The
+ * call enqueues kernels only (no allocation, no synchronization). */
+int bgx_rollout_adjoint(const bgx_model* model, int32_t n_envs, int32_t n_steps,
+                        int32_t n_frames, int32_t ckpt_every,
+                        const float* ckpt_q, const float* ckpt_qd,
+                        const float* act, int64_t act_step_stride,
+                        const float* gq_traj, const float* gqd_traj,
+                        float* gact, float* gq0, float* gqd0, float* workspace,
+                        int64_t workspace_bytes, int32_t* status,
+                        void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,6 +13,9 @@ forms.  Files:
                  1-step state of the float32 oracle
   pendulum3.npz  contact-free triple pendulum, 20 env steps
   ant.npz        ant, 1 env step (float64, and the float32 oracle)
+  humanoid.npz   synthetic humanoid, q noise U(-0.01, 0.01) with the pelvis
+                 lowered by U(0, 0.03) m (feet in contact), qd U(-0.01, 0.01),
+                 act U(-0.4, 0.4); 3 env steps (float64), 1 step (float32)
 """
 
 import os
@@ -95,8 +98,24 @@ def ant(path):
                       q1_f32=st32.q.double().numpy(), qd1_f32=st32.qd.double().numpy())
 
 
+def humanoid(path):
+  sys = S.humanoid(0)
+  n, steps = 16, 3
+  rng = np.random.default_rng(3)
+  q0 = sys.init_q + rng.uniform(-0.01, 0.01, (n, sys.q_size()))
+  q0[:, 2] -= rng.uniform(0.0, 0.03, n)
+  qd0 = rng.uniform(-0.01, 0.01, (n, sys.qd_size()))
+  acts = rng.uniform(-0.4, 0.4, (steps, n, sys.act_size()))
+  q_traj, qd_traj = _trajectory(Oracle(sys, F64, inverse="exact"), q0, qd0, acts, 5)
+  o32 = Oracle(sys, torch.float32, inverse="exact")
+  st32 = o32.env_step(o32.init(q0, qd0), acts[0], 5)
+  np.savez_compressed(path, q0=q0, qd0=qd0, acts=acts, q_traj=q_traj, qd_traj=qd_traj,
+                      q1_f32=st32.q.double().numpy(), qd1_f32=st32.qd.double().numpy())
+
+
 if __name__ == "__main__":
   quadruped(os.path.join(HERE, "quadruped.npz"))
   pendulum3(os.path.join(HERE, "pendulum3.npz"))
   ant(os.path.join(HERE, "ant.npz"))
+  humanoid(os.path.join(HERE, "humanoid.npz"))
   print("written")

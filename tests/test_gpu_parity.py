@@ -56,12 +56,14 @@ def test_gpu_available():
   assert torch.cuda.is_available()
 
 
-@pytest.mark.parametrize("name", ["quadruped", "ant", "pendulum3", "pendulum1"])
+@pytest.mark.parametrize("name", ["quadruped", "ant", "pendulum3", "pendulum1",
+                                  "humanoid"])
 def test_init_frames(name):
   """pipeline.init: x, xd from q (non-unit quaternion included), qd."""
   sys = {"quadruped": lambda: S.quadruped(0), "ant": S.ant,
          "pendulum3": lambda: S.pendulum(3),
-         "pendulum1": lambda: S.pendulum(1, motor_gear=150.0)}[name]()
+         "pendulum1": lambda: S.pendulum(1, motor_gear=150.0),
+         "humanoid": lambda: S.humanoid(0)}[name]()
   rng = np.random.default_rng(3)
   n = 256
   q0 = sys.init_q + rng.uniform(-0.3, 0.3, (n, sys.q_size()))
@@ -161,6 +163,18 @@ def test_airborne_quadruped_20_steps():
 def test_ant_one_env_step():
   sys = S.ant()
   g = np.load(os.path.join(GOLDEN, "ant.npz"))
+  st = pipeline.init(sys, _t(g["q0"]), _t(g["qd0"]))
+  st = pipeline.pipeline_step(sys, st, _t(g["acts"][0]), n_frames=5)
+  torch.cuda.synchronize()
+  assert _one_step_ok(st.q.cpu(), st.qd.cpu(), g) <= 1.0
+
+
+def test_humanoid_one_env_step():
+  """Stacked hinge triples (link type '3'), 23 dofs, 13 contact candidates;
+  several envs start with the feet in contact (more contacts than the fast
+  slots, so the overflow path runs too)."""
+  sys = S.humanoid(0)
+  g = np.load(os.path.join(GOLDEN, "humanoid.npz"))
   st = pipeline.init(sys, _t(g["q0"]), _t(g["qd0"]))
   st = pipeline.pipeline_step(sys, st, _t(g["acts"][0]), n_frames=5)
   torch.cuda.synchronize()
