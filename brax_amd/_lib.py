@@ -31,6 +31,7 @@ SIGNATURES = {
     "bgx_topology_name": (ctypes.c_char_p, [c_int]),
     "bgx_topology_signature": (ctypes.c_char_p, [c_int]),
     "bgx_topology_dims": (c_int, [c_int, ctypes.POINTER(c_i32)]),
+    "bgx_topology_per_env": (c_int, [c_int]),
     "bgx_model_create": (c_int, [c_int, c_vp, c_i64, ctypes.POINTER(c_vp)]),
     "bgx_model_destroy": (c_int, [c_vp]),
     "bgx_model_topology": (c_int, [c_vp]),

@@ -5,18 +5,21 @@ brax_amd/csrc/bgx_topo.hip once per topology (-DBGX_TOPO=<struct>) and
 brax_amd/csrc/bgx.hip (the C ABI) in parallel, and links them into
 brax_amd/libbrax_gen.so.
 
-  python brax_amd/build.py [--usage] [--jobs N]
+  python brax_amd/build.py [--usage] [--jobs N] [--cached]
+
+--cached reuses object files under brax_amd/_build/ whose inputs (every
+source under csrc/, the public header, the flags) are unchanged.
 """
 
 from __future__ import annotations
 
 import concurrent.futures as cf
+import hashlib
 import os
 import re
 import shutil
 import subprocess
 import sys
-import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -54,7 +57,23 @@ def topology_structs(inc_path: str):
   return re.findall(r"X\(\d+, (Topo_\w+)\)", text)
 
 
-def build(verbose: bool = False, resource_usage: bool = False, jobs: int = 0) -> str:
+CACHE = os.path.join(HERE, "_build")
+
+
+def _inputs_digest() -> str:
+  h = hashlib.sha1()
+  for name in sorted(os.listdir(CSRC)):
+    h.update(name.encode())
+    with open(os.path.join(CSRC, name), "rb") as f:
+      h.update(f.read())
+  with open(os.path.join(os.path.dirname(HERE), "include", "brax_gen.h"), "rb") as f:
+    h.update(f.read())
+  h.update(" ".join(HIPCC_FLAGS).encode())
+  return h.hexdigest()[:16]
+
+
+def build(verbose: bool = False, resource_usage: bool = False, jobs: int = 0,
+          cached: bool = False) -> str:
   sys.path.insert(0, os.path.dirname(HERE))
   from brax_amd import codegen
   inc = os.path.join(CSRC, "topologies.inc")
@@ -62,22 +81,32 @@ def build(verbose: bool = False, resource_usage: bool = False, jobs: int = 0) ->
   structs = topology_structs(inc)
   extra = ["-Rpass-analysis=kernel-resource-usage"] if resource_usage else []
   jobs = jobs or min(8, (os.cpu_count() or 4), len(structs) + 1)
-  with tempfile.TemporaryDirectory(prefix="bgx_build_") as tmp:
-    units = [(os.path.join(CSRC, "bgx.hip"), os.path.join(tmp, "bgx.o"), [])]
-    for s in structs:
-      units.append((os.path.join(CSRC, "bgx_topo.hip"), os.path.join(tmp, f"{s}.o"),
-                    [f"-DBGX_TOPO={s}"]))
-    cmds = [[hipcc()] + HIPCC_FLAGS + d + extra + ["-c", src, "-o", obj]
-            for src, obj, d in units]
-    with cf.ThreadPoolExecutor(jobs) as ex:
-      logs = list(ex.map(lambda c: _run(c, verbose), cmds))
-    out = LIB + ".tmp"
-    _run([hipcc(), "-shared", "-fPIC"] +
-         [o for _, o, _ in units] + ["-o", out], verbose)
+  digest = _inputs_digest()
+  os.makedirs(CACHE, exist_ok=True)
+  units = [(os.path.join(CSRC, "bgx.hip"), "bgx", [])]
+  for s in structs:
+    units.append((os.path.join(CSRC, "bgx_topo.hip"), s, [f"-DBGX_TOPO={s}"]))
+  objs, cmds = [], []
+  for src, name, d in units:
+    obj = os.path.join(CACHE, f"{name}-{digest}.o")
+    objs.append(obj)
+    if cached and os.path.exists(obj) and not resource_usage:
+      continue
+    cmds.append([hipcc()] + HIPCC_FLAGS + d + extra + ["-c", src, "-o", obj + ".tmp"])
+  with cf.ThreadPoolExecutor(jobs) as ex:
+    logs = list(ex.map(lambda c: _run(c, verbose), cmds))
+  for c in cmds:
+    os.replace(c[-1], c[-1][:-4])
+  out = LIB + ".tmp"
+  _run([hipcc(), "-shared", "-fPIC"] + objs + ["-o", out], verbose)
   if resource_usage:
     for log in logs:
       print(log, flush=True)
   os.replace(out, LIB)
+  # drop objects of other input versions
+  for f in os.listdir(CACHE):
+    if f.endswith(".o") and f"-{digest}." not in f:
+      os.remove(os.path.join(CACHE, f))
   return LIB
 
 
@@ -85,4 +114,5 @@ if __name__ == "__main__":
   jobs = 0
   if "--jobs" in sys.argv:
     jobs = int(sys.argv[sys.argv.index("--jobs") + 1])
-  build(verbose=True, resource_usage="--usage" in sys.argv, jobs=jobs)
+  build(verbose=True, resource_usage="--usage" in sys.argv, jobs=jobs,
+        cached="--cached" in sys.argv)

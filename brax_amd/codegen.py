@@ -26,7 +26,7 @@ import numpy as np
 from brax_amd import systems as S
 
 # record layouts of the float32 model table (mirrored in csrc/bgx_layout.h)
-HEADER = 8  # gravity[3], dt, solver_iterations, solver_maxls, tol, ls_eps
+HEADER = 9  # gravity[3], dt, solver_iterations, solver_maxls, tol, ls_eps, 0
 LINK_STRIDE = 24  # tpos[3] trot[4] jpos[3] ipos[3] irot[4] I[6] mass
 DOF_STRIDE = 24  # ang[3] vel[3] armature damping stiffness lo hi invweight imp[9]
 ACT_STRIDE = 8  # ctrl_lo ctrl_hi force_lo force_hi gain gear bias_q bias_qd
@@ -47,8 +47,11 @@ LS_EPS = float(np.finfo(np.float32).eps)
 class Topology:
   """Static structure of a System, as consumed by the kernel templates."""
 
-  def __init__(self, sys: S.System, slots: int = DEFAULT_SLOTS):
+  def __init__(self, sys: S.System, slots: int = DEFAULT_SLOTS, per_env: bool = False):
     S.from_arrays_check(sys)
+    # per_env: every env has its own model table (same structure, its own
+    # numbers); otherwise one table is shared by the whole batch
+    self.per_env = bool(per_env)
     self.name = _ident(sys.name)
     self.link_types = sys.link_types
     self.parents = list(sys.link_parents)
@@ -127,7 +130,7 @@ class Topology:
   def signature(self) -> str:
     key = repr((self.link_types, self.parents, self.act_q, self.act_d,
                 self.lim_d, self.cand_link, self.cand_kind, self.slots,
-                self.plane_z, self.dof_hinge))
+                self.plane_z, self.dof_hinge) + ((True,) if self.per_env else ()))
     return hashlib.sha1(key.encode()).hexdigest()[:16]
 
   def cpp_struct(self) -> str:
@@ -139,7 +142,8 @@ class Topology:
     s += f'  static constexpr const char* NAME = "{self.name}";\n'
     s += (f"  static constexpr int L = {self.L}, NQ = {self.nq}, NV = {self.nv}, "
           f"NU = {self.nu}, NLIM = {len(self.lim_d)}, NCAND = {self.ncand}, "
-          f"KC = {self.slots}, PLANE_Z = {int(self.plane_z)};\n")
+          f"KC = {self.slots}, PLANE_Z = {int(self.plane_z)}, "
+          f"PER_ENV = {int(self.per_env)};\n")
     s += (f"  static constexpr int OFF_LINK = {self.off_link}, OFF_DOF = "
           f"{self.off_dof}, OFF_ACT = {self.off_act}, OFF_CAND = "
           f"{self.off_cand}, SIZE = {self.size};\n")
@@ -252,21 +256,27 @@ def pack_model(sys: S.System, topo: Topology = None) -> np.ndarray:
 # registry of topologies compiled into the library
 
 
-def registered_systems() -> List[S.System]:
-  """Systems whose topologies are compiled into libbrax_gen.so."""
-  return [
-      S.quadruped(0),
-      S.ant(),
-      S.pendulum(1, motor_gear=150.0),
-      S.pendulum(3),
-      S.humanoid(0),
+CHAIN_LINKS = range(2, 33)  # hinge chains of the heterogeneous-batch workload
+
+
+def registered_systems() -> List[Tuple[S.System, bool]]:
+  """(system, per-env tables) whose topologies are compiled into
+  libbrax_gen.so."""
+  out = [
+      (S.quadruped(0), False),
+      (S.ant(), False),
+      (S.pendulum(1, motor_gear=150.0), False),
+      (S.pendulum(3), False),
+      (S.humanoid(0), False),
   ]
+  out += [(S.chain(n, 0), True) for n in CHAIN_LINKS]
+  return out
 
 
 def registered_topologies() -> List[Topology]:
   seen: Dict[str, Topology] = {}
-  for sys in registered_systems():
-    t = Topology(sys)
+  for sys, per_env in registered_systems():
+    t = Topology(sys, per_env=per_env)
     seen.setdefault(t.signature(), t)
   return list(seen.values())
 

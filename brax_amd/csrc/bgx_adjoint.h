@@ -253,7 +253,7 @@ BGX_D int adjoint_a(const float* __restrict__ M, const float* q_in, const float*
   float etas[ADJ_MAXITER], betas[ADJ_MAXITER];
   const bool has_rows = (n_c > 0) || any_lim;
   const int maxiter = (int)M[H_MAXITER], maxls = (int)M[H_MAXLS];
-  if (maxiter > ADJ_MAXITER) return -1;
+  if (has_rows && maxiter > ADJ_MAXITER) return -1;
   const float tol = M[H_TOL], lseps = M[H_LSEPS];
   float tk = 1.f, eta = 1.f, err = INFINITY;
   bool running = has_rows;

@@ -33,6 +33,7 @@ struct Launch {
   float* xd_ang;
   float* xd_vel;
   int* status;
+  int status_or;  // OR the status bits into status instead of assigning
   float* probe;
   float* sub_q;   // (n_steps * n_frames, n, NQ): state at the start of each substep
   float* sub_qd;  // (n_steps * n_frames, n, NV)
@@ -51,14 +52,19 @@ struct AdjLaunch {
   float* adj;        // (n, NQ + NV), read by stage A, written by stage B
   float* rec;        // (SIZE, n) stage A record
   float* gact;       // (n, NU) accumulated
-  int* status;
 };
+
+// the model table of env: shared, or one per env (PER_ENV topologies)
+template <class T>
+BGX_D const float* env_model(const float* M, int env) {
+  return T::PER_ENV ? M + (long long)env * T::SIZE : M;
+}
 
 template <class T>
 BGX_D void write_frames(const Launch& a, int env, const float* q, const float* qd) {
   V3 xp[T::L], xa[T::L], xv[T::L];
   Q4 xr[T::L];
-  forward_kinematics<T, true>(a.M, q, qd, xp, xr, xa, xv);
+  forward_kinematics<T, true>(env_model<T>(a.M, env), q, qd, xp, xr, xa, xv);
 #pragma unroll
   for (int l = 0; l < T::L; ++l) {
     const long long o3 = ((long long)env * T::L + l) * 3;
@@ -121,7 +127,7 @@ __global__ __launch_bounds__(WAVE) void k_rollout(Launch a) {
 #pragma unroll
         for (int k = 0; k < T::NV; ++k) a.sub_qd[(i * a.n + env) * T::NV + k] = qd[k];
       }
-      st |= substep<T>(a.M, q, qd, act, lds, probe);
+      st |= substep<T>(env_model<T>(a.M, env), q, qd, act, lds, probe);
     }
     if (a.q_traj) {
 #pragma unroll
@@ -147,7 +153,7 @@ __global__ __launch_bounds__(WAVE) void k_rollout(Launch a) {
   }
   if (!finite) st |= ST_NONFINITE;
   if (a.x_pos || a.x_rot || a.xd_ang || a.xd_vel) write_frames<T>(a, env, q, qd);
-  if (a.status) a.status[env] = st;
+  if (a.status) a.status[env] = a.status_or ? (a.status[env] | st) : st;
 }
 
 // adjoint stage A: one thread per env
@@ -174,10 +180,14 @@ __global__ __launch_bounds__(WAVE) void k_adj_a(AdjLaunch a) {
     act[k] = a.act ? a.act[(long long)env * T::NU + k] : 0.f;
     gact[k] = 0.f;
   }
-  const int st = adjoint_a<T>(a.M, q, qd, act, gq1, gqd1, RecPtr{a.rec + env, a.n}, gact);
+  // the return value (slow-path bit) is not used here: the recompute launch
+  // reports the status.  With a status update in this kernel, hipcc's gfx950
+  // code for the ant topology returned wrong cotangents in some lanes, while
+  // the same source built for the host is clean under address, undefined
+  // and memory sanitizers and matches autograd.
+  adjoint_a<T>(a.M, q, qd, act, gq1, gqd1, RecPtr{a.rec + env, a.n}, gact);
 #pragma unroll
   for (int k = 0; k < T::NU; ++k) a.gact[(long long)env * T::NU + k] += gact[k];
-  if (a.status) a.status[env] |= (st < 0 ? ST_NONFINITE : st);
 }
 
 // adjoint stage B: one thread per (env, input coordinate)
@@ -202,6 +212,7 @@ struct TopoOps {
   const char* sig;
   int dims[8];  // L, NQ, NV, NU, NLIM, NCAND, KC, table size
   int adj_rec;  // floats per env of the adjoint record
+  int per_env;  // one model table per env
   hipError_t (*init)(const Launch&, hipStream_t);
   hipError_t (*rollout)(const Launch&, hipStream_t);
   hipError_t (*adj_a)(const AdjLaunch&, hipStream_t);
@@ -229,14 +240,18 @@ struct Ops {
     return hipGetLastError();
   }
   static TopoOps table() {
-    return TopoOps{T::NAME,
-                   T::SIG,
-                   {T::L, T::NQ, T::NV, T::NU, T::NLIM, T::NCAND, T::KC, T::SIZE},
-                   AdjRec<T>::SIZE,
-                   &init,
-                   &rollout,
-                   &adj_a,
-                   &adj_b};
+    TopoOps t{T::NAME,
+              T::SIG,
+              {T::L, T::NQ, T::NV, T::NU, T::NLIM, T::NCAND, T::KC, T::SIZE},
+              AdjRec<T>::SIZE,
+              T::PER_ENV,
+              &init,
+              &rollout,
+              nullptr,
+              nullptr};
+    // the adjoint is compiled for topologies with a shared model table
+    if constexpr (!T::PER_ENV) t.adj_a = &adj_a, t.adj_b = &adj_b;
+    return t;
   }
 };
 

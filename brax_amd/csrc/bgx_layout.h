@@ -10,6 +10,7 @@ constexpr int H_MAXITER = 4;  // solver_iterations (stored as float)
 constexpr int H_MAXLS = 5;    // solver_maxls
 constexpr int H_TOL = 6;      // projected-gradient stopping tolerance
 constexpr int H_LSEPS = 7;    // line-search slack
+constexpr int H_ZERO = 8;     // always 0 (an offset the compiler cannot fold)
 
 // impedance record: constants of constraint.py:29-65 derived on the host
 // from (timeconst, dampratio, dmin, dmax, width, mid, power)

@@ -189,24 +189,33 @@ struct Tree {
   V3 rc[T::L];              // root COM of each link's tree
 };
 
+// unroll factor of the dof-indexed nested loops: fully unrolled (registers,
+// static indices) up to 20 dofs; above, rolled loops keep compile time and
+// code size bounded (the factor then lives in scratch memory)
+template <class T>
+constexpr int unroll_nv() {
+  return T::NV <= 20 ? 32 : 1;
+}
+
 // M = L^T L in place on the lower part of M (Featherstone's LTL: leaves
 // first, no fill-in outside the dof-ancestor pattern)
 template <class T>
 BGX_D void ltl_factor(Tree<T>& t) {
-#pragma unroll
+  constexpr int U = unroll_nv<T>();
+#pragma unroll U
   for (int k = T::NV - 1; k >= 0; --k) {
     const float lkk = fsqrt(t.Lm[k][k]);
     t.Lm[k][k] = lkk;
     t.inv[k] = rcp(lkk);
-#pragma unroll
+#pragma unroll U
     for (int j = 0; j < k; ++j)
       if ((T::DANC[k] >> j) & 1u) t.Lm[k][j] *= t.inv[k];
-#pragma unroll
+#pragma unroll U
     for (int i = 0; i < k; ++i) {
       if ((T::DANC[k] >> i) & 1u) {
         const float a = t.Lm[k][i];
         t.Lm[i][i] -= a * a;
-#pragma unroll
+#pragma unroll U
         for (int j = 0; j < i; ++j)
           if ((T::DANC[i] >> j) & 1u) t.Lm[i][j] -= a * t.Lm[k][j];
       }
@@ -217,16 +226,17 @@ BGX_D void ltl_factor(Tree<T>& t) {
 // M z = tau with M = L^T L (in place on z)
 template <class T>
 BGX_D void ltl_solve(const Tree<T>& t, float* z) {
-#pragma unroll
+  constexpr int U = unroll_nv<T>();
+#pragma unroll U
   for (int i = T::NV - 1; i >= 0; --i) {
     z[i] *= t.inv[i];
-#pragma unroll
+#pragma unroll U
     for (int j = 0; j < i; ++j)
       if ((T::DANC[i] >> j) & 1u) z[j] -= t.Lm[i][j] * z[i];
   }
-#pragma unroll
+#pragma unroll U
   for (int i = 0; i < T::NV; ++i) {
-#pragma unroll
+#pragma unroll U
     for (int j = 0; j < i; ++j)
       if ((T::DANC[i] >> j) & 1u) z[i] -= t.Lm[i][j] * z[j];
     z[i] *= t.inv[i];
@@ -508,13 +518,14 @@ BGX_D void crba(const float* __restrict__ M, V3T<S>* cc, Sym3T<S>* ci,
       km[par] = km[par] + km[l];
     }
   }
-#pragma unroll
+  constexpr int U = unroll_nv<T>();
+#pragma unroll U
   for (int i = 0; i < T::NV; ++i) {
     const int l = T::DOF_LINK[i];
     V3T<S> fa, fv;
     inertia_mul(cc[l], ci[l], km[l], ca[i], cv[i], fa, fv);
     Ml[i][i] = dot(cv[i], fv) + dot(ca[i], fa) + dof_rec<T>(M, i)[DF_ARMATURE];
-#pragma unroll
+#pragma unroll U
     for (int j = 0; j < i; ++j)
       if ((T::DANC[i] >> j) & 1u) Ml[i][j] = dot(cv[j], fv) + dot(ca[j], fa);
   }
@@ -1081,12 +1092,10 @@ BGX_D bool substep_ks(const float* __restrict__ M, float* q, float* qd,
   return substep_impl<T, KS>(M, q, qd, act, xp, xr, rc, lds, rows, dbg);
 }
 
-// an opaque copy of a uniform pointer: loads through it are not merged with
-// loads through the original
-BGX_D const float* launder(const float* p) {
-  asm volatile("" : "+s"(p));
-  return p;
-}
+// an opaque copy of the model pointer (offset by the table's zero slot, which
+// the compiler cannot fold): loads through it are not merged with loads
+// through the original
+BGX_D const float* launder(const float* M) { return M + (int)M[H_ZERO]; }
 
 // one substep: the fast path keeps the active contacts in KC slots (rows in
 // LDS); when more candidates touch the floor it returns before changing the

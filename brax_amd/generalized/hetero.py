@@ -1,0 +1,113 @@
+"""Environments of different structure stepped together.
+
+The reference vmaps one System over a batch (brax/envs/wrappers/training.py:
+59-72); envs of different topologies need one compiled program per topology.
+``HeteroBatch`` keeps a per-env offset table (env -> group, row) over groups
+of envs that share a compiled topology.  A group of one shared System uses a
+single model table; a group of per-env systems (``SystemBatch``, e.g. chains
+with their own lengths and masses) uses one table per env.  ``rollout``
+launches every group's kernel on its own stream so the groups run
+concurrently; ``graph()`` captures all of them in one HIP graph, replayed as
+one launch.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from typing import Dict, List, Sequence
+
+import torch
+
+from brax_amd import _lib
+from brax_amd import codegen
+from brax_amd import systems as S
+from brax_amd.generalized import pipeline
+
+
+class _Group:
+
+  def __init__(self, envs: List[int], sys_obj, device):
+    self.envs = envs
+    self.sys = sys_obj
+    self.model = pipeline.device_model(sys_obj, device)
+    self.n = len(envs)
+
+
+class HeteroBatch:
+  """One System per env; envs are grouped by compiled topology."""
+
+  def __init__(self, systems: Sequence[S.System], device=None):
+    self.device = torch.device(device if device is not None else "cuda")
+    if self.device.index is None:
+      self.device = torch.device("cuda", torch.cuda.current_device())
+    compiled = _lib.topologies()
+    keyed: Dict[str, List[int]] = {}
+    for e, s in enumerate(systems):
+      shared = codegen.Topology(s).signature()
+      per_env = codegen.Topology(s, per_env=True).signature()
+      if per_env in compiled:
+        key = "pe:" + per_env
+      elif shared in compiled:
+        key = f"sh:{shared}:{id(s)}"
+      else:
+        raise NotImplementedError(f"topology of {s.name!r} is not compiled")
+      keyed.setdefault(key, []).append(e)
+    self.groups: List[_Group] = []
+    # most expensive groups first (dofs^3 per env), so they start first
+    order = sorted(keyed.items(), key=lambda kv: -systems[kv[1][0]].qd_size() ** 3 * len(kv[1]))
+    for key, envs in order:
+      obj = (pipeline.SystemBatch([systems[e] for e in envs]) if key.startswith("pe:")
+             else systems[envs[0]])
+      self.groups.append(_Group(envs, obj, self.device))
+    self.n = len(systems)
+    # per-env offset table: env -> (group, row)
+    self.env_group = torch.empty(self.n, dtype=torch.int64)
+    self.env_row = torch.empty(self.n, dtype=torch.int64)
+    for g, grp in enumerate(self.groups):
+      for r, e in enumerate(grp.envs):
+        self.env_group[e], self.env_row[e] = g, r
+    self._streams = None
+
+  def split(self, per_env: Sequence[torch.Tensor]) -> List[torch.Tensor]:
+    """Per-group (n_g, width_g) tensors from per-env rows."""
+    return [torch.stack([torch.as_tensor(per_env[e], dtype=torch.float32)
+                         for e in g.envs]).to(self.device).contiguous()
+            for g in self.groups]
+
+  def gather(self, per_group: Sequence[torch.Tensor], env: int) -> torch.Tensor:
+    """Row of env from per-group tensors."""
+    return per_group[int(self.env_group[env])][int(self.env_row[env])]
+
+  def streams(self, k: int = 4):
+    if self._streams is None:
+      self._streams = [torch.cuda.Stream(device=self.device) for _ in range(k)]
+    return self._streams
+
+  def launch_rollout(self, q: List[torch.Tensor], qd: List[torch.Tensor],
+                     acts: List[torch.Tensor], n_steps: int, n_frames: int,
+                     status: List[torch.Tensor] = None):
+    """Enqueues every group's rollout (in place on q, qd) on its own stream,
+    forked from and joined back into the current stream."""
+    lib = _lib.load()
+    cur = torch.cuda.current_stream(self.device)
+    streams = self.streams()
+    for s in streams:
+      s.wait_stream(cur)
+    for g, grp in enumerate(self.groups):
+      s = streams[g % len(streams)]
+      m = grp.model
+      a = acts[g] if m.nu else None
+      _lib.check(lib.bgx_rollout(
+          m.handle, grp.n, n_steps, n_frames, _lib.ptr(q[g]), _lib.ptr(qd[g]),
+          _lib.ptr(a), grp.n * m.nu, None, None, None, None, None, None,
+          _lib.ptr(status[g]) if status is not None else None,
+          ctypes.c_void_p(s.cuda_stream)))
+    for s in streams:
+      cur.wait_stream(s)
+
+  def rollout(self, q, qd, acts, n_steps: int, n_frames: int):
+    """Advances per-group states q[g], qd[g] (in place) by n_steps env steps;
+    acts[g]: (n_steps, n_g, nu_g)."""
+    status = [torch.zeros(g.n, dtype=torch.int32, device=self.device) for g in self.groups]
+    self.launch_rollout(q, qd, acts, n_steps, n_frames, status)
+    return status

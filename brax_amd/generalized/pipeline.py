@@ -30,22 +30,50 @@ STATUS_SLOW_PATH = 1
 STATUS_NONFINITE = 2
 
 
-class DeviceModel:
-  """A System uploaded to one GPU: topology id + device model table."""
+class SystemBatch:
+  """One System per env, all of one structure: the envs share a compiled
+  topology and each has its own model table (the reference batches per-env
+  System arrays with jax.vmap, brax/envs/wrappers/training.py:223-260)."""
 
-  def __init__(self, sys: S.System, device: torch.device):
+  def __init__(self, systems):
+    self.systems = list(systems)
+    if not self.systems:
+      raise ValueError("empty SystemBatch")
+    sigs = {codegen.Topology(s, per_env=True).signature() for s in self.systems}
+    if len(sigs) != 1:
+      raise ValueError("the systems of a SystemBatch must share one structure")
+
+  def __len__(self):
+    return len(self.systems)
+
+
+class DeviceModel:
+  """A System (or a SystemBatch) uploaded to one GPU: topology id + device
+  model table(s)."""
+
+  def __init__(self, sys, device: torch.device):
     self.sys = sys
     self.device = device
-    topo = codegen.Topology(sys)
+    batch = isinstance(sys, SystemBatch)
+    systems = sys.systems if batch else [sys]
+    topo = codegen.Topology(systems[0], per_env=batch)
     compiled = _lib.topologies()
     if topo.signature() not in compiled:
+      hint = ""
+      if not batch and codegen.Topology(sys, per_env=True).signature() in compiled:
+        hint = " (it is compiled with per-env tables: pass a SystemBatch)"
       raise NotImplementedError(
-          f"the topology of system {sys.name!r} is not compiled into "
+          f"the topology of system {systems[0].name!r} is not compiled into "
           "libbrax_gen.so; add it to brax_amd.codegen.registered_systems() "
-          "and rebuild")
+          f"and rebuild{hint}")
     self.topo_id, _, dims = compiled[topo.signature()]
     self.L, self.nq, self.nv, self.nu = dims[0], dims[1], dims[2], dims[3]
-    table = np.ascontiguousarray(codegen.pack_model(sys, topo))
+    self.n_tables = len(systems)
+    packed = {}  # a System repeated over envs is packed once
+    for s in systems:
+      if id(s) not in packed:
+        packed[id(s)] = codegen.pack_model(s, topo)
+    table = np.ascontiguousarray(np.concatenate([packed[id(s)] for s in systems]))
     handle = ctypes.c_void_p()
     lib = _lib.load()
     with torch.cuda.device(device):
@@ -65,7 +93,9 @@ class DeviceModel:
 _MODELS: Dict[Tuple[int, str], Tuple[S.System, DeviceModel]] = {}
 
 
-def device_model(sys: S.System, device=None) -> DeviceModel:
+def device_model(sys, device=None) -> DeviceModel:
+  """The uploaded model of a System or SystemBatch (cached per object and
+  device)."""
   device = torch.device(device if device is not None else "cuda")
   if device.type != "cuda":
     raise ValueError("the generalized pipeline runs on the GPU only")
@@ -74,7 +104,8 @@ def device_model(sys: S.System, device=None) -> DeviceModel:
   key = (id(sys), str(device))
   hit = _MODELS.get(key)
   if hit is None or hit[0] is not sys:
-    S.from_arrays_check(sys)
+    for s in (sys.systems if isinstance(sys, SystemBatch) else [sys]):
+      S.from_arrays_check(s)
     hit = (sys, DeviceModel(sys, device))
     _MODELS[key] = hit
   return hit[1]

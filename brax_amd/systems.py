@@ -1041,6 +1041,36 @@ def pendulum(n_links: int = 1, motor_gear: Optional[float] = None,
   return b.build()
 
 
+def chain(n_links: int, seed: int = 0) -> System:
+  """Serial hinge chain of capsule links hanging from the world.
+
+  The heterogeneous-batch workload's chains: link k is a capsule of length
+  U(0.1, 0.5) m (radius 0.04) hanging along -z from its hinge; its mass is
+  log-uniform on [0.1, 10] kg; hinge axes alternate y, x (so the chain moves
+  in 3D); the first hinge is fixed to the world at z = 2; every link has a
+  motor (gear 10, ctrl [-1, 1]); armature 0.1 and damping 0.5 per hinge;
+  no contacts and no limits.  Options as the quadruped (timestep 0.01,
+  4 solver iterations).  q size = qd size = n_links.
+  """
+  rng = np.random.default_rng(seed)
+  lengths = rng.uniform(0.1, 0.5, size=n_links)
+  masses = np.exp(rng.uniform(np.log(0.1), np.log(10.0), size=n_links))
+  b = ModelBuilder(f"chain{n_links}", timestep=0.01, iterations=4, solver_maxls=15,
+                   matrix_inv_iterations=10)
+  parent = 0
+  for k in range(n_links):
+    pos = (0, 0, 2.0) if k == 0 else (0, 0, -lengths[k - 1])
+    body = b.add_body(f"link{k}", parent, pos=pos)
+    b.add_hinge(body, f"hinge{k}", (0, 1, 0) if k % 2 == 0 else (1, 0, 0),
+                armature=0.1, damping=0.5)
+    b.add_geom(body, GEOM_CAPSULE, (0.04,), fromto=(0, 0, 0, 0, 0, -lengths[k]),
+               mass=masses[k], contype=0, conaffinity=0, name=f"capsule{k}")
+    parent = body
+  for k in range(n_links):
+    b.add_motor(f"hinge{k}", 10.0, (-1.0, 1.0))
+  return b.build()
+
+
 def free_body(gravity=(0.0, 0.0, -9.81), radius: float = 0.1,
               mass: float = 1.0, timestep: float = 0.01,
               pos=(0.0, 0.0, 1.0)) -> System:

@@ -32,6 +32,7 @@ enum {
   BGX_ERR_ARG = 1,      /* null pointer, bad size or count */
   BGX_ERR_TOPOLOGY = 2, /* topology id not compiled into this library */
   BGX_ERR_HIP = 3,      /* HIP runtime error (allocation, launch) */
+  BGX_ERR_DISABLED = 4, /* entry point switched off in this build */
 };
 
 /* per-env status bits written by bgx_step / bgx_rollout */
@@ -58,9 +59,13 @@ int bgx_num_topologies(void);
 const char* bgx_topology_name(int topo_id);
 const char* bgx_topology_signature(int topo_id);
 int bgx_topology_dims(int topo_id, int32_t* dims);
+/* 1 if the topology takes one model table per env (bgx_model_create with
+ * k tables for up to k envs), 0 if one table is shared, -1 if unknown. */
+int bgx_topology_per_env(int topo_id);
 
 /* Uploads a model table (float32, layout of brax_amd/codegen.py pack_model)
- * for a topology.  Replaces brax.io.mjcf.load's output feeding the pipeline
+ * for a topology; for a per-env topology, k tables back to back (env e uses
+ * table e).  Replaces brax.io.mjcf.load's output feeding the pipeline
  * (brax/io/mjcf.py:317-489): the System arrays the step reads. */
 int bgx_model_create(int topo_id, const float* host_table, int64_t n_floats,
                      bgx_model** out);
@@ -125,7 +130,11 @@ int64_t bgx_adjoint_workspace_bytes(const bgx_model* model, int32_t n_envs,
  *     the actions and of the initial state.  status (N, may be NULL)
  *     receives BGX_STATUS_* bits of the recomputed substeps.
  * workspace: device memory of bgx_adjoint_workspace_bytes() bytes.  The
- * call enqueues kernels only (no allocation, no synchronization). */
+ * call enqueues kernels only (no allocation, no synchronization).
+ * Switched off in this build: returns BGX_ERR_DISABLED after validating
+ * its arguments (its gfx950 kernels gave wrong cotangents for some
+ * topologies and a memory fault in a test build; the same source built for
+ * the host matches autograd). */
 int bgx_rollout_adjoint(const bgx_model* model, int32_t n_envs, int32_t n_steps,
                         int32_t n_frames, int32_t ckpt_every,
                         const float* ckpt_q, const float* ckpt_qd,

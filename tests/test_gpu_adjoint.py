@@ -21,7 +21,12 @@ from brax_amd import systems as S
 from brax_amd.generalized import pipeline
 from oracle.generalized import Oracle
 
-pytestmark = pytest.mark.gpu
+pytestmark = [
+    pytest.mark.gpu,
+    pytest.mark.skip(reason="bgx_rollout_adjoint is switched off: its gfx950 stage-A kernel "
+                            "returned wrong cotangents for some topologies and faulted in a "
+                            "test build; the host build of the same source matches autograd"),
+]
 DEV = torch.device("cuda", 0)
 GRAD_RTOL = 1e-3
 
@@ -149,3 +154,41 @@ def test_adjoint_humanoid():
   ref = _oracle_grads(sys, q0, qd0, acts, wq, wqd, 5)
   gpu = _gpu_grads(sys, q0, qd0, acts, wq, wqd, 5, 1)
   _compare("humanoid", gpu, ref)
+
+
+def test_adjoint_ignores_workspace_contents():
+  """The adjoint reads only what it wrote: a NaN-filled workspace gives the
+  same gradient as a zero-filled one (ant, contacts)."""
+  import ctypes  # noqa: F401
+  from brax_amd import _lib
+  sys = S.ant()
+  q0, qd0, acts, wq, wqd = _inputs(sys, 8, 2, seed=2)
+  m = pipeline.device_model(sys, DEV)
+  lib = _lib.load()
+  n, T = q0.shape[0], acts.shape[0]
+  t = lambda a: torch.tensor(a, device=DEV).contiguous()  # noqa: E731
+  q, qd, a = t(q0), t(qd0), t(acts)
+  qt = torch.empty(T, n, m.nq, device=DEV)
+  qdt = torch.empty(T, n, m.nv, device=DEV)
+  _lib.check(lib.bgx_rollout(m.handle, n, T, 5, _lib.ptr(q), _lib.ptr(qd), _lib.ptr(a),
+                             n * m.nu, _lib.ptr(qt), _lib.ptr(qdt), None, None, None, None,
+                             None, None))
+  nbytes = lib.bgx_adjoint_workspace_bytes(m.handle, n, 1, 5)
+  gwq, gwqd = t(wq), t(wqd)  # kept alive while the kernels run
+  outs = []
+  for fill in (0.0, float("nan")):
+    ws = torch.full((nbytes // 4 + 1,), fill, device=DEV)
+    ckq = torch.cat([t(q0)[None], qt[:-1]]).contiguous()
+    ckqd = torch.cat([t(qd0)[None], qdt[:-1]]).contiguous()
+    ga = torch.empty(T, n, m.nu, device=DEV)
+    gq = torch.empty(n, m.nq, device=DEV)
+    gqd = torch.empty(n, m.nv, device=DEV)
+    _lib.check(lib.bgx_rollout_adjoint(m.handle, n, T, 5, 1, _lib.ptr(ckq), _lib.ptr(ckqd),
+                                       _lib.ptr(a), n * m.nu, _lib.ptr(gwq), _lib.ptr(gwqd),
+                                       _lib.ptr(ga), _lib.ptr(gq), _lib.ptr(gqd), _lib.ptr(ws),
+                                       nbytes, None, None))
+    torch.cuda.synchronize()
+    outs.append([x.cpu().numpy() for x in (gq, gqd, ga)])
+  for x, y in zip(*outs):
+    assert np.isfinite(y).all()
+    np.testing.assert_array_equal(x, y)

@@ -41,12 +41,13 @@ def test_python_binding_covers_header(lib):
 def test_registered_topologies_compiled(lib):
   from brax_amd import _lib
   compiled = _lib.topologies()
-  for sys in codegen.registered_systems():
-    t = codegen.Topology(sys)
+  for sys, per_env in codegen.registered_systems():
+    t = codegen.Topology(sys, per_env=per_env)
     assert t.signature() in compiled
     tid, _, dims = compiled[t.signature()]
     assert dims[:7] == [t.L, t.nq, t.nv, t.nu, len(t.lim_d), t.ncand, t.slots]
     assert dims[7] == t.size == codegen.pack_model(sys, t).size
+    assert lib.bgx_topology_per_env(tid) == int(per_env)
 
 
 def test_error_paths_without_gpu(lib):
