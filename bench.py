@@ -87,6 +87,18 @@ def workload(name, n, seed=0):
   return sys_, q0, qd0, rng
 
 
+def measured_traffic(name, n, steps):
+  """HBM bytes per launch of `steps` env steps from the committed rocprofv3
+  summary of this workload (FETCH_SIZE + WRITE_SIZE, calibrated), or None."""
+  path = os.path.join(ROOT, "profiles", f"k_rollout_{name}_{n}.md")
+  if not os.path.exists(path):
+    return None
+  for line in open(path):
+    if line.startswith("traffic_bytes_per_env_step = "):
+      return float(line.split("=")[1]) * steps
+  return None
+
+
 def cpu_baseline(sys_, q0, qd0, act_range, rng_seed=1, n=256, steps=8):
   """Oracle (float32, torch CPU) on the first n envs for `steps` env steps:
   env-steps/s on this host, and the mean active constraint-row count."""
@@ -173,6 +185,13 @@ def forward_bench(args, name):
   wall = time.perf_counter() - t0
   kernel_ms = ev0.elapsed_time(ev1)
 
+  if getattr(args, "calib_copy", False):
+    src = torch.ones(64 << 20, device=dev)
+    dst = torch.empty_like(src)
+    torch.add(src, 0.0, out=dst)  # reads 256 MiB, writes 256 MiB
+    torch.cuda.synchronize()
+    del src, dst
+
   st = status.cpu().numpy()
   finite = bool(torch.isfinite(q).all() and torch.isfinite(qd).all())
   env_steps_per_s = n * K / wall
@@ -180,8 +199,8 @@ def forward_bench(args, name):
   cpu = None
   r_a = 9.9
   if not args.no_cpu:
-    cpu, r_a = cpu_baseline(sys_, q0, qd0, act_range,
-                            n=256 if name == "quadruped" else 64)
+    cpu, r_a = cpu_baseline(sys_, q0, qd0, act_range, n=256 if name == "quadruped" else 64,
+                            steps=64 if name == "quadruped" else 48)
   nv, L = sys_.qd_size(), sys_.num_links()
   fl = flop_per_substep(L, nv, sys_.num_contacts(), sys_.num_limit_rows(),
                         len(sys_.geom_type), sys_.solver_iterations, r_a)
@@ -209,7 +228,8 @@ def forward_bench(args, name):
                  "qd_size": nv, "parallelism": "none (1 GPU)"},
       "roofline": {"bound": "valu", "achieved": achieved,
                    "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                   "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                   "frac": achieved / FP32_PEAK_TFLOPS,
+                   "traffic": measured_traffic(name, n, K),
                    "flop_per_substep": fl, "mean_active_rows": r_a,
                    "kernel_ms": kernel_ms,
                    "hbm_algorithmic_gbs": hbm_bytes / (kernel_ms * 1e-3) / 1e9},
@@ -395,6 +415,9 @@ def main():
   ap.add_argument("--ckpt", type=int, default=16)
   ap.add_argument("--no-graph", action="store_true")
   ap.add_argument("--no-cpu", action="store_true")
+  ap.add_argument("--calib-copy", action="store_true",
+                  help="run a 256 MiB elementwise copy before the timed region "
+                       "(a known-size kernel for scaling HBM counters)")
   args = ap.parse_args()
   if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
     print("bench.py measures one GPU: the step has no cross-GPU exchange; run "
