@@ -30,28 +30,6 @@ namespace bgx {
 
 constexpr int ADJ_MAXITER = 8;  // most solver iterations the adjoint unrolls
 
-constexpr int popc(unsigned x) {
-  int n = 0;
-  while (x) {
-    n += (int)(x & 1u);
-    x >>= 1;
-  }
-  return n;
-}
-// index of entry (i, j), j <= i, of the lower dof-ancestor pattern: rows in
-// order, in each row the ancestor columns ascending, then the diagonal
-template <class T>
-constexpr int mpat_row(int i) {
-  int s = 0;
-  for (int r = 0; r < i; ++r) s += popc(T::DANC[r]) + 1;
-  return s;
-}
-template <class T>
-constexpr int mpat_idx(int i, int j) {
-  return mpat_row<T>(i) +
-         (j == i ? popc(T::DANC[i]) : popc(T::DANC[i] & ((1u << j) - 1u)));
-}
-
 template <class T>
 struct AdjRec {
   static constexpr int NV = T::NV, NQ = T::NQ;
@@ -102,6 +80,8 @@ BGX_D int adjoint_a(const float* __restrict__ M, const float* q_in, const float*
   forward_kinematics<T, false, float>(M, q, qd, xp, xr, nullptr, nullptr);
   const V3 rc = root_com<T>(M, xp, xr);
   Tree<T> t;
+  float lm_store[T::LM_LDS ? R::NMB : 1];
+  if constexpr (T::LM_LDS) t.lml = lm_store, t.lms = 1;
 #pragma unroll
   for (int l = 0; l < L; ++l) t.rc[l] = rc;
 
@@ -152,7 +132,7 @@ BGX_D int adjoint_a(const float* __restrict__ M, const float* q_in, const float*
   float qf[NV];
   rnea_bias<T>(M, t.ca, t.cv, cda, cdv, cc, ci, qd, qf);
   smooth_forces<T>(M, q, qd, act, qf);
-  crba<T>(M, cc, ci, t.ca, t.cv, t.Lm);
+  crba<T>(M, cc, ci, t.ca, t.cv, [&](int i, int j, float v) { t.lm(i, j) = v; });
   ltl_factor<T>(t);
   float u[NV];
 #pragma unroll
@@ -525,7 +505,7 @@ BGX_D float adjoint_b(const float* __restrict__ M, const float* q_in, const floa
   for (int d = 0; d < NV; ++d) phi += rec[R::QFB + d] * qf[d].d;
   {
     S Ml[NV][NV];
-    crba<T>(M, cc, ci, ca, cv, Ml);
+    crba<T>(M, cc, ci, ca, cv, [&](int i, int j, S v) { Ml[i][j] = v; });
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
       phi += rec[R::MB + mpat_idx<T>(i, i)] * Ml[i][i].d;

@@ -99,11 +99,7 @@ __global__ __launch_bounds__(WAVE) void k_init(Launch a) {
 
 template <class T>
 __global__ __launch_bounds__(WAVE) void k_rollout(Launch a) {
-  // link scratch (6 per link), then the fast-path slot rows (4 vectors x 4
-  // rows), which also hold a second link vector while the rows are unused
-  constexpr int kRowsArea =
-      RV_N * 4 * T::KC > T::L * 6 ? RV_N * 4 * T::KC : T::L * 6;
-  __shared__ float smem[(T::L * 6 + kRowsArea) * WAVE];
+  __shared__ float smem[lds_floats<T>() * WAVE];  // layout: lds_floats()
   const int env = blockIdx.x * WAVE + threadIdx.x;
   if (env >= a.n) return;
   float* lds = smem + threadIdx.x;

@@ -181,13 +181,80 @@ BGX_D void forward_kinematics(const float* __restrict__ M, const S* q, const S* 
 // ---------------------------------------------------------------------------
 // per-substep tree quantities
 
+constexpr int popc(unsigned x) {
+  int n = 0;
+  while (x) {
+    n += (int)(x & 1u);
+    x >>= 1;
+  }
+  return n;
+}
+// index of entry (i, j), j <= i, of the lower dof-ancestor pattern: rows in
+// order, in each row the ancestor columns ascending, then the diagonal
+template <class T>
+constexpr int mpat_row(int i) {
+  int s = 0;
+  for (int r = 0; r < i; ++r) s += popc(T::DANC[r]) + 1;
+  return s;
+}
+template <class T>
+struct MPat {
+  struct Table {
+    int v[T::NV][T::NV];
+  };
+  static constexpr Table make() {
+    Table t{};
+    for (int i = 0; i < T::NV; ++i)
+      for (int j = 0; j <= i; ++j)
+        t.v[i][j] = mpat_row<T>(i) +
+                    (j == i ? popc(T::DANC[i]) : popc(T::DANC[i] & ((1u << j) - 1u)));
+    return t;
+  }
+  static constexpr Table table = make();
+};
+// (a table lookup, so that unrolled loops fold it to a constant)
+template <class T>
+BGX_D constexpr int mpat_idx(int i, int j) {
+  return MPat<T>::table.v[i][j];
+}
+
 template <class T>
 struct Tree {
+  static constexpr int NR = T::LM_LDS ? 1 : T::NV;
   V3 ca[T::NV], cv[T::NV];  // cdof (ang, vel)
-  float Lm[T::NV][T::NV];   // LTL factor, lower part on the dof-ancestor pattern
+  // LTL factor on the lower dof-ancestor pattern: in registers, or (LM_LDS
+  // topologies) packed by mpat_idx at lml[idx * lms] (LDS or private memory)
+  float Lm[NR][NR];
+  float* lml = nullptr;
+  int lms = WAVE;
   float inv[T::NV];         // 1 / L_kk
   V3 rc[T::L];              // root COM of each link's tree
+  BGX_D float& lm(int i, int j) {
+    if constexpr (T::LM_LDS) return lml[mpat_idx<T>(i, j) * lms];
+    else return Lm[i][j];
+  }
+  BGX_D float lm(int i, int j) const {
+    if constexpr (T::LM_LDS) return lml[mpat_idx<T>(i, j) * lms];
+    else return Lm[i][j];
+  }
 };
+
+// LDS of the step kernel per lane (floats, stride WAVE): link scratch (6 per
+// link), the fast-path slot rows (4 vectors x 4 rows per slot; also a second
+// link vector while the rows are unused), then for LM_LDS topologies the LTL
+// factor and the 4 limit-row solver vectors
+template <class T>
+constexpr int lds_rows_area() {
+  return 4 * 4 * T::KC > T::L * 6 ? 4 * 4 * T::KC : T::L * 6;
+}
+template <class T>
+constexpr int lds_lm_offset() {
+  return T::L * 6 + lds_rows_area<T>();
+}
+template <class T>
+constexpr int lds_floats() {
+  return lds_lm_offset<T>() + (T::LM_LDS ? mpat_row<T>(T::NV) + 4 * T::NLIM : 0);
+}
 
 // unroll factor of the dof-indexed nested loops: fully unrolled (registers,
 // static indices) up to 20 dofs; above, rolled loops keep compile time and
@@ -204,20 +271,20 @@ BGX_D void ltl_factor(Tree<T>& t) {
   constexpr int U = unroll_nv<T>();
 #pragma unroll U
   for (int k = T::NV - 1; k >= 0; --k) {
-    const float lkk = fsqrt(t.Lm[k][k]);
-    t.Lm[k][k] = lkk;
+    const float lkk = fsqrt(t.lm(k, k));
+    t.lm(k, k) = lkk;
     t.inv[k] = rcp(lkk);
 #pragma unroll U
     for (int j = 0; j < k; ++j)
-      if ((T::DANC[k] >> j) & 1u) t.Lm[k][j] *= t.inv[k];
+      if ((T::DANC[k] >> j) & 1u) t.lm(k, j) *= t.inv[k];
 #pragma unroll U
     for (int i = 0; i < k; ++i) {
       if ((T::DANC[k] >> i) & 1u) {
-        const float a = t.Lm[k][i];
-        t.Lm[i][i] -= a * a;
+        const float a = t.lm(k, i);
+        t.lm(i, i) -= a * a;
 #pragma unroll U
         for (int j = 0; j < i; ++j)
-          if ((T::DANC[i] >> j) & 1u) t.Lm[i][j] -= a * t.Lm[k][j];
+          if ((T::DANC[i] >> j) & 1u) t.lm(i, j) -= a * t.lm(k, j);
       }
     }
   }
@@ -232,13 +299,13 @@ BGX_D void ltl_solve(const Tree<T>& t, float* z) {
     z[i] *= t.inv[i];
 #pragma unroll U
     for (int j = 0; j < i; ++j)
-      if ((T::DANC[i] >> j) & 1u) z[j] -= t.Lm[i][j] * z[i];
+      if ((T::DANC[i] >> j) & 1u) z[j] -= t.lm(i, j) * z[i];
   }
 #pragma unroll U
   for (int i = 0; i < T::NV; ++i) {
 #pragma unroll U
     for (int j = 0; j < i; ++j)
-      if ((T::DANC[i] >> j) & 1u) z[i] -= t.Lm[i][j] * z[j];
+      if ((T::DANC[i] >> j) & 1u) z[i] -= t.lm(i, j) * z[j];
     z[i] *= t.inv[i];
   }
 }
@@ -503,9 +570,9 @@ BGX_D void smooth_forces(const float* __restrict__ M, const S* q, const S* qd,
 // mass.matrix (mass.py:27-83): composite rigid-body inertias (cc, ci turn into
 // subtree sums in place), then M_ij = cdof_j . (I_c(i) cdof_i) on the
 // dof-ancestor pattern, lower part, with armature on the diagonal
-template <class T, class S>
+template <class T, class S, class Put>
 BGX_D void crba(const float* __restrict__ M, V3T<S>* cc, Sym3T<S>* ci,
-                const V3T<S>* ca, const V3T<S>* cv, S (*Ml)[T::NV]) {
+                const V3T<S>* ca, const V3T<S>* cv, Put put) {
   float km[T::L];
 #pragma unroll
   for (int l = 0; l < T::L; ++l) km[l] = link_rec<T>(M, l)[LK_MASS];
@@ -524,10 +591,10 @@ BGX_D void crba(const float* __restrict__ M, V3T<S>* cc, Sym3T<S>* ci,
     const int l = T::DOF_LINK[i];
     V3T<S> fa, fv;
     inertia_mul(cc[l], ci[l], km[l], ca[i], cv[i], fa, fv);
-    Ml[i][i] = dot(cv[i], fv) + dot(ca[i], fa) + dof_rec<T>(M, i)[DF_ARMATURE];
+    put(i, i, dot(cv[i], fv) + dot(ca[i], fa) + dof_rec<T>(M, i)[DF_ARMATURE]);
 #pragma unroll U
     for (int j = 0; j < i; ++j)
-      if ((T::DANC[i] >> j) & 1u) Ml[i][j] = dot(cv[j], fv) + dot(ca[j], fa);
+      if ((T::DANC[i] >> j) & 1u) put(i, j, dot(cv[j], fv) + dot(ca[j], fa));
   }
 }
 
@@ -708,6 +775,21 @@ struct LdsRows {
     return base[((v * KS + s) * 4 + k) * WAVE];
   }
 };
+// limit-row solver vectors x, y, g, r (k = 0..3): registers, or LDS for
+// LM_LDS topologies (element (k, i) of this lane at base[(k NLIM + i) WAVE])
+template <class T, bool LDS>
+struct LimVecs;
+template <class T>
+struct LimVecs<T, false> {
+  float v[4][T::NLIM > 0 ? T::NLIM : 1];
+  BGX_D float& at(int k, int i) { return v[k][i]; }
+};
+template <class T>
+struct LimVecs<T, true> {
+  float* base;
+  BGX_D float& at(int k, int i) { return base[(k * T::NLIM + i) * WAVE]; }
+};
+
 template <int KS>
 struct PrivRows {
   float a[RV_N][KS][4];
@@ -727,6 +809,7 @@ BGX_D bool substep_impl(const float* __restrict__ M, float* q, float* qd,
   constexpr int NL = T::NLIM > 0 ? T::NLIM : 1;
   constexpr bool PZ = T::PLANE_Z != 0;
   Tree<T> t;
+  if constexpr (T::LM_LDS) t.lml = lds + lds_lm_offset<T>() * WAVE;
   float* lds2 = lds + L * 6 * WAVE;  // second link-vector area (rows space)
 #pragma unroll
   for (int l = 0; l < L; ++l) t.rc[l] = rc0;
@@ -806,7 +889,7 @@ BGX_D bool substep_impl(const float* __restrict__ M, float* q, float* qd,
 
   // ---- mass.matrix: CRBA on the dof-ancestor pattern (mass.py:27-83), then
   // the LTL factorization, leaves first
-  crba<T>(M, cc, ci, t.ca, t.cv, t.Lm);
+  crba<T>(M, cc, ci, t.ca, t.cv, [&](int i, int j, float v) { t.lm(i, j) = v; });
   ltl_factor<T>(t);
 
   // ---- unconstrained acceleration u = M^-1 qf_smooth
@@ -952,9 +1035,13 @@ BGX_D bool substep_impl(const float* __restrict__ M, float* q, float* qd,
   };
 
   // ---- jaxopt.ProjectedGradient: FISTA + backtracking line search from 0
-  float lx[NL], ly[NL], lg[NL], lr[NL];
+  LimVecs<T, T::LM_LDS != 0> lv;
+  if constexpr (T::LM_LDS)
+    lv.base = lds + (lds_lm_offset<T>() + mpat_row<T>(T::NV)) * WAVE;
 #pragma unroll
-  for (int i = 0; i < NL; ++i) lx[i] = ly[i] = lg[i] = lr[i] = 0.f;
+  for (int i = 0; i < T::NLIM; ++i)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) lv.at(k, i) = 0.f;
 
   const bool has_rows = (n_c > 0) || any_lim;
   const int maxiter = (int)M[H_MAXITER], maxls = (int)M[H_MAXLS];
@@ -968,7 +1055,7 @@ BGX_D bool substep_impl(const float* __restrict__ M, float* q, float* qd,
     // gradient at y: r = A y + b, g = A^T r
     float fy = 0.f;
     apply_a([&](int s, int k) { return rows.at(RV_Y, s, k); },
-            [&](int i) { return ly[i]; },
+            [&](int i) { return lv.at(1, i); },
             [&](int s, int k, float o) {
               const float r = o + sb[s][k];
               rows.at(RV_R, s, k) = r;
@@ -976,13 +1063,13 @@ BGX_D bool substep_impl(const float* __restrict__ M, float* q, float* qd,
             },
             [&](int i, float o) {
               const float r = o + lb[i];
-              lr[i] = r;
+              lv.at(3, i) = r;
               fy += 0.5f * r * r;
             });
     apply_a([&](int s, int k) { return rows.at(RV_R, s, k); },
-            [&](int i) { return lr[i]; },
+            [&](int i) { return lv.at(3, i); },
             [&](int s, int k, float o) { rows.at(RV_G, s, k) = o; },
-            [&](int i, float o) { lg[i] = o; });
+            [&](int i, float o) { lv.at(2, i) = o; });
     // line search: trial x+ = relu(y - eta g), residual into r
     bool cond = true;
     for (int ls = 0; ls <= maxls && cond; ++ls) {
@@ -992,7 +1079,7 @@ BGX_D bool substep_impl(const float* __restrict__ M, float* q, float* qd,
       apply_a([&](int s, int k) {
                 return fmaxf(rows.at(RV_Y, s, k) - e_ * rows.at(RV_G, s, k), 0.f);
               },
-              [&](int i) { return fmaxf(ly[i] - e_ * lg[i], 0.f); },
+              [&](int i) { return fmaxf(lv.at(1, i) - e_ * lv.at(2, i), 0.f); },
               [&](int s, int k, float o) {
                 const float r = o + sb[s][k];
                 rows.at(RV_R, s, k) = r;
@@ -1004,10 +1091,10 @@ BGX_D bool substep_impl(const float* __restrict__ M, float* q, float* qd,
               },
               [&](int i, float o) {
                 const float r = o + lb[i];
-                lr[i] = r;
+                lv.at(3, i) = r;
                 fx += 0.5f * r * r;
-                const float df = fmaxf(ly[i] - e_ * lg[i], 0.f) - ly[i];
-                dg += df * lg[i];
+                const float df = fmaxf(lv.at(1, i) - e_ * lv.at(2, i), 0.f) - lv.at(1, i);
+                dg += df * lv.at(2, i);
                 dd += df * df;
               });
       trials += 1;
@@ -1017,14 +1104,14 @@ BGX_D bool substep_impl(const float* __restrict__ M, float* q, float* qd,
     float e2 = 0.f;
     const float e_ = eta;
     apply_a([&](int s, int k) { return rows.at(RV_R, s, k); },
-            [&](int i) { return lr[i]; },
+            [&](int i) { return lv.at(3, i); },
             [&](int s, int k, float o) {
               const float xp = fmaxf(rows.at(RV_Y, s, k) - e_ * rows.at(RV_G, s, k), 0.f);
               const float dv = fmaxf(xp - o, 0.f) - xp;
               e2 += dv * dv;
             },
             [&](int i, float o) {
-              const float xp = fmaxf(ly[i] - e_ * lg[i], 0.f);
+              const float xp = fmaxf(lv.at(1, i) - e_ * lv.at(2, i), 0.f);
               const float dv = fmaxf(xp - o, 0.f) - xp;
               e2 += dv * dv;
             });
@@ -1042,9 +1129,9 @@ BGX_D bool substep_impl(const float* __restrict__ M, float* q, float* qd,
         }
 #pragma unroll
     for (int i = 0; i < T::NLIM; ++i) {
-      const float xp = fmaxf(ly[i] - e_ * lg[i], 0.f);
-      ly[i] = xp + beta * (xp - lx[i]);
-      lx[i] = xp;
+      const float xp = fmaxf(lv.at(1, i) - e_ * lv.at(2, i), 0.f);
+      lv.at(1, i) = xp + beta * (xp - lv.at(0, i));
+      lv.at(0, i) = xp;
     }
     eta = eta <= 1e-6f ? 1.f : eta * 2.f;
     tk = tn;
@@ -1055,7 +1142,7 @@ BGX_D bool substep_impl(const float* __restrict__ M, float* q, float* qd,
   float qdd[NV];
   if (has_rows) {
     apply_jt([&](int s, int k) { return rows.at(RV_X, s, k); },
-             [&](int i) { return lx[i]; }, qdd);
+             [&](int i) { return lv.at(0, i); }, qdd);
     ltl_solve<T>(t, qdd);
 #pragma unroll
     for (int d = 0; d < NV; ++d) qdd[d] = u[d] + qdd[d];
@@ -1105,9 +1192,10 @@ BGX_D int substep(const float* __restrict__ M, float* q, float* qd,
                   const float* act, float* lds, float* dbg) {
   LdsRows<T::KC> rows{lds + T::L * 6 * WAVE};
   if (substep_ks<T, T::KC>(M, q, qd, act, lds, rows, dbg)) return 0;
-  constexpr int KB = T::NCAND > T::KC ? T::NCAND : T::KC;
-  PrivRows<KB> prow;
-  substep_ks<T, KB>(launder(M), q, qd, act, lds, prow, dbg);
+  if constexpr (T::KC < T::NCAND) {
+    PrivRows<T::NCAND> prow;
+    substep_ks<T, T::NCAND>(launder(M), q, qd, act, lds, prow, dbg);
+  }
   return ST_SLOW_PATH;
 }
 
