@@ -184,9 +184,8 @@ int bgx_step_probe(const bgx_model* m, int32_t n, int32_t n_frames, float* q, fl
 
 namespace {
 
-// the adjoint kernels are not launched until their gfx950 code is shown
-// correct on every compiled topology
-constexpr bool kAdjointEnabled = false;
+// the adjoint's GPU path (on: tape-based stage A1/A2 kernels)
+constexpr bool kAdjointEnabled = true;
 
 struct AdjWorkspace {
   float* sub_q;   // (k * n_frames, n, NQ)
@@ -239,7 +238,7 @@ int bgx_rollout_adjoint(const bgx_model* m, int32_t n, int32_t n_steps, int32_t 
   if (!valid_topo(m->topo)) return fail(BGX_ERR_TOPOLOGY, "unknown topology id");
   if (n < 0 || n_steps < 0 || n_frames < 1 || ckpt_every < 1)
     return fail(BGX_ERR_ARG, "bad env, step, frame or checkpoint count");
-  if (!kOps[m->topo]->adj_a)
+  if (!kOps[m->topo]->adj_fwd)
     return fail(BGX_ERR_TOPOLOGY, "the adjoint is not compiled for per-env topologies");
   // (a topology without contact candidates or limits never runs the solver)
   const bool has_rows = kOps[m->topo]->dims[4] > 0 || kOps[m->topo]->dims[5] > 0;
@@ -254,11 +253,7 @@ int bgx_rollout_adjoint(const bgx_model* m, int32_t n, int32_t n_steps, int32_t 
   if (nu > 0 && !gact) return fail(BGX_ERR_ARG, "gact is null");
   const AdjWorkspace w = adj_layout(ops, workspace, n, ckpt_every, n_frames);
   if (workspace_bytes < w.bytes) return fail(BGX_ERR_ARG, "workspace too small");
-  if (!kAdjointEnabled)
-    return fail(BGX_ERR_DISABLED,
-                "bgx_rollout_adjoint is switched off in this build: its gfx950 stage-A "
-                "kernel returned wrong cotangents for some topologies and faulted in a "
-                "test build, while the same source built for the host matches autograd");
+  if (!kAdjointEnabled) return fail(BGX_ERR_DISABLED, "bgx_rollout_adjoint is switched off");
   hipStream_t s = static_cast<hipStream_t>(stream);
   int rc;
   const long long nact = (long long)n_steps * n * nu, nadj = (long long)n * (nq + nv);
@@ -297,7 +292,8 @@ int bgx_rollout_adjoint(const bgx_model* m, int32_t n, int32_t n_steps, int32_t 
       a.gqd_add = (step_end && gqd_traj) ? gqd_traj + (long long)t * n * nv : nullptr;
       a.adj = w.adj, a.rec = w.rec;
       a.gact = gact ? gact + (long long)t * n * nu : nullptr;
-      if ((rc = check_hip(ops->adj_a(a, s), "adjoint stage A launch"))) return rc;
+      if ((rc = check_hip(ops->adj_fwd(a, s), "adjoint stage A1 launch"))) return rc;
+      if ((rc = check_hip(ops->adj_rev(a, s), "adjoint stage A2 launch"))) return rc;
       if ((rc = check_hip(ops->adj_b(a, s), "adjoint stage B launch"))) return rc;
     }
   }

@@ -8,313 +8,144 @@
 // derivative, and the smooth arithmetic between them is differentiated.
 // This file computes the same derivative in two stages:
 //
-//  A (one thread per env): recompute the substep, keeping the solver
-//    iterates, then run the reverse sweep of the integrator, the solves with
-//    M = L^T L and the unrolled FISTA iterations by hand.  Output: cotangents
-//    of the position- and velocity-dependent ingredients -- qf_smooth, the
-//    mass matrix (a symmetric matrix on the dof-ancestor pattern), the
-//    contact rows of J, the diagonal regularizer d and aref -- plus the
-//    action gradient and the velocity pass-through of qd' = qd + dt qdd.
+//  A1 (one thread per env): the step's own substep (substep_impl, a slot per
+//    candidate) recording its tape: tree, contact slots, accelerations and
+//    every projected-gradient iterate.
+//  A2 (one thread per env): the reverse sweep of the integrator, the solves
+//    with M = L^T L and the unrolled FISTA iterations, by hand, from the
+//    tape.  Output: cotangents of the position- and velocity-dependent
+//    ingredients -- qf_smooth, the mass matrix (a symmetric matrix on the
+//    dof-ancestor pattern), the contact rows of J, the diagonal regularizer
+//    d and aref -- plus the action gradient and the velocity pass-through of
+//    qd' = qd + dt qdd.
 //  B (one thread per env and input coordinate): run the same formulas that
 //    build those ingredients from (q, qd) in dual numbers along one input
 //    coordinate and contract the tangents with the cotangents of stage A:
 //    d/de <cotangents, ingredients(q + e dq, qd + e dqd)>.
 //
-// Record of stage A for stage B: one float per field per env, field-major
-// (field f of env e at rec[f * n_envs + e]).
+// Record of stages A1/A2 for stage B: one float per field per env,
+// field-major (field f of env e at rec[f * n_envs + e]).
 #pragma once
 
 #include "bgx_step.h"
 
 namespace bgx {
 
-constexpr int ADJ_MAXITER = 8;  // most solver iterations the adjoint unrolls
-
+// record of one env: the forward's tape (TapeLayout), then the cotangents
+// stage A2 writes for stage B
 template <class T>
-struct AdjRec {
-  static constexpr int NV = T::NV, NQ = T::NQ;
-  static constexpr int KS = T::NCAND > 0 ? T::NCAND : 1;  // one slot per candidate
-  static constexpr int NRC = 4 * KS;                      // contact rows
-  static constexpr int NLR = T::NLIM;                     // limit rows
-  static constexpr int NR = NRC + (NLR > 0 ? NLR : 1);
-  static constexpr int NMB = mpat_row<T>(T::NV);
-  static constexpr int QFB = 0;               // [NV]  cotangent of qf_smooth
-  static constexpr int MB = QFB + NV;         // [NMB] cotangent of M (symmetrized)
-  static constexpr int JB = MB + NMB;         // [NRC][NV] cotangent of J contact rows
-  static constexpr int DB = JB + NRC * NV;    // [NR]  cotangent of d
-  static constexpr int AB = DB + NR;          // [NR]  cotangent of aref
-  static constexpr int GQ1 = AB + NR;         // [NQ]  cotangent of q'
-  static constexpr int QD1 = GQ1 + NQ;        // [NV]  qd'
-  static constexpr int GQDT = QD1 + NV;       // [NV]  qd' cotangent incl. the q' update
-  static constexpr int CSLOT = GQDT + NV;     // [KS]  slot of each candidate, -1 inactive
-  static constexpr int CFB = CSLOT + KS;      // [KS]  capsule frame fallback taken
-  static constexpr int SIZE = CFB + KS;
+struct AdjRec : TapeLayout<T> {
+  using TL = TapeLayout<T>;
+  static constexpr int QFB = TL::SIZE;               // [NV]  cotangent of qf_smooth
+  static constexpr int MB = QFB + TL::NV;            // [NMB] cotangent of M (symmetrized)
+  static constexpr int JB = MB + TL::NMB;            // [NRC][NV] cotangent of J contact rows
+  static constexpr int DB = JB + TL::NRC * TL::NV;   // [NR]  cotangent of d
+  static constexpr int AB = DB + TL::NR;             // [NR]  cotangent of aref
+  static constexpr int GQ1 = AB + TL::NR;            // [NQ]  cotangent of q'
+  static constexpr int GQDT = GQ1 + TL::NQ;          // [NV]  qd' cotangent incl. the q' update
+  static constexpr int SIZE = GQDT + TL::NV;
 };
 
-struct RecPtr {
-  float* p;
-  long long n;
-  BGX_D float& operator[](int f) const { return p[(long long)f * n]; }
-};
+BGX_D V3 ld3(const RecPtr& rec, int f) { return v3(rec[f], rec[f + 1], rec[f + 2]); }
 
 // ---------------------------------------------------------------------------
-// stage A
+// stage A2: reverse sweep from the tape (stage A1 is substep_impl recording
+// its tape with a slot per candidate).  Slot loops are rolled (dynamic slot
+// index, tape in global memory): compact code for any candidate count.
 
 template <class T>
-BGX_D int adjoint_a(const float* __restrict__ M, const float* q_in, const float* qd_in,
-                    const float* act, const float* gq1, const float* gqd1, RecPtr rec,
-                    float* gact) {
+BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float* qd,
+                       const float* act, const float* gq1, const float* gqd1, RecPtr rec,
+                       float* gact) {
   using R = AdjRec<T>;
-  constexpr int NV = T::NV, NQ = T::NQ, L = T::L, KS = R::KS, NR = R::NR;
-  constexpr int NRC = R::NRC;
-  constexpr bool PZ = T::PLANE_Z != 0;
-  float q[NQ], qd[NV];
-#pragma unroll
-  for (int k = 0; k < NQ; ++k) q[k] = q_in[k];
-#pragma unroll
-  for (int k = 0; k < NV; ++k) qd[k] = qd_in[k];
+  constexpr int NV = T::NV, L = T::L, NR = R::NR, NRC = R::NRC;
+  const int n_c = (int)rec[R::NC], iters = (int)rec[R::ITERS];
 
-  // ---- recompute the substep, as substep_impl with a slot per candidate
-  V3 xp[L];
-  Q4 xr[L];
-  forward_kinematics<T, false, float>(M, q, qd, xp, xr, nullptr, nullptr);
-  const V3 rc = root_com<T>(M, xp, xr);
   Tree<T> t;
   float lm_store[T::LM_LDS ? R::NMB : 1];
   if constexpr (T::LM_LDS) t.lml = lm_store, t.lms = 1;
 #pragma unroll
-  for (int l = 0; l < L; ++l) t.rc[l] = rc;
-
-  SlotGeo<PZ> sg[KS];
-  float sdiag[KS], sbq[KS], skt[KS];
-  int n_c = 0;
+  for (int d = 0; d < NV; ++d) {
+    t.ca[d] = ld3(rec, R::CA + 3 * d);
+    t.cv[d] = ld3(rec, R::CV + 3 * d);
+    t.inv[d] = rec[R::INV + d];
+    t.lm(d, d) = rec[R::LM + mpat_idx<T>(d, d)];
 #pragma unroll
-  for (int c = 0; c < T::NCAND; ++c) {
-    V3 ctr, axis;
-    candidate_center<T>(M, c, xp, xr, ctr, axis);
-    const float dist = candidate_dist<T>(M, c, ctr);
-    int slot = -1, fb = 0;
-    if (dist < 0.f) {
-      const int l = T::CAND_LINK[c];
-      const float* cd = cand_rec<T>(M, c);
-      V3 pos, t1, t2;
-      fb = contact_frame<T>(M, c, ctr, axis, dist, -1, pos, t1, t2) ? 1 : 0;
-      const V3 n = PZ ? v3(0.f, 0.f, 1.f) : ldv3(cd + CD_PN);
-      const float mu = cd[CD_MU];
-      SlotGeo<PZ> g;
-      slot_set(g, l, pos - rc, n, t1 * mu, t2 * mu);
-      const float* im = cd + CD_IMP;
-      const float imp = impedance(im, dist);
-      const float tw = cd[CD_TW];
-      const float diag = (tw + mu * mu * tw) * (2.f * mu * mu * imp_ratio(imp));
-#pragma unroll
-      for (int ss = 0; ss < KS; ++ss) {
-        if (ss <= c && ss == n_c) {
-          sg[ss] = g;
-          sdiag[ss] = diag;
-          sbq[ss] = im[IM_B];
-          skt[ss] = im[IM_K] * imp * dist;
-        }
-      }
-      slot = n_c;
-      n_c += 1;
-    }
-    rec[R::CSLOT + c] = (float)slot;
-    rec[R::CFB + c] = (float)fb;
+    for (int j = 0; j < d; ++j)
+      if ((T::DANC[d] >> j) & 1u) t.lm(d, j) = rec[R::LM + mpat_idx<T>(d, j)];
   }
-
-  V3 cc[L];
-  Sym3 ci[L];
-  com_inertia<T>(M, xp, xr, rc, cc, ci);
-  dof_motions<T>(M, q, xp, xr, rc, t.ca, t.cv);
-  V3 cda[L], cdv[L];
-  link_vel<T>(t.ca, t.cv, qd, cda, cdv);
-  float qf[NV];
-  rnea_bias<T>(M, t.ca, t.cv, cda, cdv, cc, ci, qd, qf);
-  smooth_forces<T>(M, q, qd, act, qf);
-  crba<T>(M, cc, ci, t.ca, t.cv, [&](int i, int j, float v) { t.lm(i, j) = v; });
-  ltl_factor<T>(t);
-  float u[NV];
+  float lside[R::NLR], ldiag[R::NLR];
+  bool any_lim = false;
 #pragma unroll
-  for (int d = 0; d < NV; ++d) u[d] = qf[d];
-  ltl_solve<T>(t, u);
+  for (int i = 0; i < T::NLIM; ++i) {
+    lside[i] = rec[R::LSIDE + i];
+    ldiag[i] = rec[R::LDIAG + i];
+    any_lim = any_lim || lside[i] != 0.f;
+  }
+  const bool has_rows = n_c > 0 || any_lim;
 
-  // rows: contact rows 4 s + k for slots s < n_c, then limit rows NRC + i
-  float lside[NR], rdiag[NR], b[NR];
-#pragma unroll
-  for (int r = 0; r < NR; ++r) lside[r] = rdiag[r] = b[r] = 0.f;
-
+  // row vectors: contact rows 4 s + k (s < n_c), then limit rows NRC + i
   auto J_apply = [&](const float* z, float* out) {
     V3 va[L], vv[L];
     link_vel<T>(t.ca, t.cv, z, va, vv);
+#pragma unroll 1
+    for (int s = 0; s < n_c; ++s) {
+      const int l = (int)rec[R::SLINK + s];
+      V3 a = va[0], v = vv[0];
 #pragma unroll
-    for (int s = 0; s < KS; ++s) {
-      float e0 = 0.f, e1 = 0.f, e2 = 0.f;
-      if (s < n_c) {
-        V3 a = va[0], v = vv[0];
-#pragma unroll
-        for (int l = 1; l < L; ++l)
-          if (sg[s].link == l) a = va[l], v = vv[l];
-        slot_dots(sg[s], v + cross(a, sg[s].r), e0, e1, e2);
-      }
+      for (int k = 1; k < L; ++k)
+        if (l == k) a = va[k], v = vv[k];
+      const V3 vp = v + cross(a, ld3(rec, R::SR + 3 * s));
+      const float e0 = dot(ld3(rec, R::SD0 + 3 * s), vp);
+      const float e1 = dot(ld3(rec, R::SD1 + 3 * s), vp);
+      const float e2 = dot(ld3(rec, R::SD2 + 3 * s), vp);
       out[4 * s] = e0 + e1;
       out[4 * s + 1] = e0 - e1;
       out[4 * s + 2] = e0 + e2;
       out[4 * s + 3] = e0 - e2;
     }
+#pragma unroll 1
+    for (int r = 4 * n_c; r < NRC; ++r) out[r] = 0.f;
 #pragma unroll
-    for (int i = 0; i < T::NLIM; ++i) out[NRC + i] = lside[NRC + i] * z[T::LIM_D[i]];
+    for (int i = 0; i < T::NLIM; ++i) out[NRC + i] = lside[i] * z[T::LIM_D[i]];
   };
   auto JT_apply = [&](const float* x, float* tau) {
     V3 fa[L], fv[L];
 #pragma unroll
     for (int l = 0; l < L; ++l) fa[l] = v3zero(), fv[l] = v3zero();
+#pragma unroll 1
+    for (int s = 0; s < n_c; ++s) {
+      const float x0 = x[4 * s], x1 = x[4 * s + 1], x2 = x[4 * s + 2], x3 = x[4 * s + 3];
+      const V3 w = ld3(rec, R::SD0 + 3 * s) * ((x0 + x1) + (x2 + x3)) +
+                   ld3(rec, R::SD1 + 3 * s) * (x0 - x1) + ld3(rec, R::SD2 + 3 * s) * (x2 - x3);
+      const V3 ta = cross(ld3(rec, R::SR + 3 * s), w);
+      const int l = (int)rec[R::SLINK + s];
 #pragma unroll
-    for (int s = 0; s < KS; ++s) {
-      if (s < n_c) {
-        const V3 w = slot_force(sg[s], x[4 * s], x[4 * s + 1], x[4 * s + 2], x[4 * s + 3]);
-        const V3 ta = cross(sg[s].r, w);
-#pragma unroll
-        for (int l = 0; l < L; ++l)
-          if (hosts_contact<T>(l) && sg[s].link == l) fa[l] = fa[l] + ta, fv[l] = fv[l] + w;
-      }
+      for (int k = 0; k < L; ++k)
+        if (hosts_contact<T>(k) && l == k) fa[k] = fa[k] + ta, fv[k] = fv[k] + w;
     }
     project_forces<T>(t, fa, fv, tau);
 #pragma unroll
-    for (int i = 0; i < T::NLIM; ++i) tau[T::LIM_D[i]] += lside[NRC + i] * x[NRC + i];
+    for (int i = 0; i < T::NLIM; ++i) tau[T::LIM_D[i]] += lside[i] * x[NRC + i];
+  };
+  auto rdiag = [&](int r) {
+    return r < NRC ? (r < 4 * n_c ? rec[R::SDIAG + r / 4] : 0.f) : ldiag[r - NRC];
   };
   // out = A x, ux = M^-1 J^T x
   auto A_apply = [&](const float* x, float* out, float* ux) {
     JT_apply(x, ux);
     ltl_solve<T>(t, ux);
     J_apply(ux, out);
-#pragma unroll
-    for (int r = 0; r < NR; ++r) out[r] = out[r] + rdiag[r] * x[r];
+#pragma unroll 1
+    for (int r = 0; r < NR; ++r) out[r] = out[r] + rdiag(r) * x[r];
+  };
+  auto row = [&](int base, int k, float* out) {  // a tape row vector of iteration k
+#pragma unroll 1
+    for (int r = 0; r < NR; ++r) out[r] = rec[base + k * NR + r];
   };
 
-  {
-    float ju[NR], jq[NR];
-    J_apply(u, ju);
-    J_apply(qd, jq);
-#pragma unroll
-    for (int s = 0; s < KS; ++s)
-      if (s < n_c)
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          b[4 * s + k] = ju[4 * s + k] + (sbq[s] * jq[4 * s + k] + skt[s]);
-          rdiag[4 * s + k] = sdiag[s];
-        }
-  }
-  bool any_lim = false;
-#pragma unroll
-  for (int i = 0; i < T::NLIM; ++i) {
-    const int d = T::LIM_D[i], qi = T::LIM_Q[i];
-    const float* df = dof_rec<T>(M, d);
-    const float pmin = q[qi] - df[DF_LO];
-    const float pmax = df[DF_HI] - q[qi];
-    const float pos = fminf(fminf(pmin, pmax), 0.f);
-    const bool on = pos < 0.f;
-    const float side = on ? (pmin < pmax ? 1.f : -1.f) : 0.f;
-    const float* im = df + DF_IMP;
-    const float imp = impedance(im, pos);
-    const float vel = side * qd[d];
-    const float aref = on ? (-im[IM_B] * vel - im[IM_K] * imp * pos) : 0.f;
-    lside[NRC + i] = side;
-    rdiag[NRC + i] = on ? df[DF_INVWEIGHT] * imp_ratio(imp) : 0.f;
-    b[NRC + i] = side * u[d] - aref;
-    any_lim = any_lim || on;
-  }
-
-  // ---- projected gradient, keeping y_k, r_k = A y_k + b, x_{k+1}, eta_k, beta_k
-  float X[NR], Y[NR], G[NR], Rr[NR];
-#pragma unroll
-  for (int r = 0; r < NR; ++r) X[r] = Y[r] = G[r] = Rr[r] = 0.f;
-  float Ys[ADJ_MAXITER][NR], Rs[ADJ_MAXITER][NR], Xs[ADJ_MAXITER][NR];
-  float etas[ADJ_MAXITER], betas[ADJ_MAXITER];
-  const bool has_rows = (n_c > 0) || any_lim;
-  const int maxiter = (int)M[H_MAXITER], maxls = (int)M[H_MAXLS];
-  if (has_rows && maxiter > ADJ_MAXITER) return -1;
-  const float tol = M[H_TOL], lseps = M[H_LSEPS];
-  float tk = 1.f, eta = 1.f, err = INFINITY;
-  bool running = has_rows;
-  int iters = 0;
-  for (int it = 0; it < maxiter; ++it) {
-    if (it > 0) running = running && (err > tol);
-    if (!running) break;
-    float tmp[NV], ar[NR];
-    A_apply(Y, ar, tmp);
-    float fy = 0.f;
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-      Rr[r] = ar[r] + b[r];
-      fy += 0.5f * Rr[r] * Rr[r];
-    }
-#pragma unroll
-    for (int r = 0; r < NR; ++r) Ys[it][r] = Y[r], Rs[it][r] = Rr[r];
-    A_apply(Rr, G, tmp);
-    bool cond = true;
-    for (int ls = 0; ls <= maxls && cond; ++ls) {
-      if (ls > 0) eta *= 0.5f;
-      const float e_ = eta;
-      float xt[NR];
-#pragma unroll
-      for (int r = 0; r < NR; ++r) xt[r] = fmaxf(Y[r] - e_ * G[r], 0.f);
-      A_apply(xt, ar, tmp);
-      float fx = 0.f, dg = 0.f, dd = 0.f;
-#pragma unroll
-      for (int r = 0; r < NR; ++r) {
-        const float rr = ar[r] + b[r];
-        Rr[r] = rr;
-        fx += 0.5f * rr * rr;
-        const float df = xt[r] - Y[r];
-        dg += df * G[r];
-        dd += df * df;
-      }
-      cond = e_ * (fx - fy) > e_ * dg + 0.5f * dd + lseps;
-    }
-    const float e_ = eta;
-    float xn[NR], gx[NR];
-#pragma unroll
-    for (int r = 0; r < NR; ++r) xn[r] = fmaxf(Y[r] - e_ * G[r], 0.f);
-    A_apply(Rr, gx, tmp);
-    float e2 = 0.f;
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-      const float dv = fmaxf(xn[r] - gx[r], 0.f) - xn[r];
-      e2 += dv * dv;
-    }
-    err = fsqrt(e2);
-    const float tn = 0.5f * (1.f + fsqrt(1.f + 4.f * tk * tk));
-    const float beta = (tk - 1.f) * rcp(tn);
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-      Xs[it][r] = xn[r];
-      Y[r] = xn[r] + beta * (xn[r] - X[r]);
-      X[r] = xn[r];
-    }
-    etas[it] = e_;
-    betas[it] = beta;
-    eta = eta <= 1e-6f ? 1.f : eta * 2.f;
-    tk = tn;
-    iters += 1;
-  }
-
-  // qdd = u + w, w = M^-1 J^T x; qd' = qd + qdd dt
-  float w[NV];
-  if (has_rows) {
-    JT_apply(X, w);
-    ltl_solve<T>(t, w);
-  } else {
-#pragma unroll
-    for (int d = 0; d < NV; ++d) w[d] = 0.f;
-  }
+  // ---- q' = f(q, qd'): cotangent of qd' from the position update
   const float dt = M[H_DT];
-  float qd1[NV];
-#pragma unroll
-  for (int d = 0; d < NV; ++d) qd1[d] = qd[d] + (u[d] + w[d]) * dt;
-
-  // ---- reverse sweep
-  // q' = f(q, qd'): cotangent of qd' from the position update
   float gqdt[NV];
 #pragma unroll
   for (int d = 0; d < NV; ++d) gqdt[d] = gqd1[d];
@@ -327,9 +158,9 @@ BGX_D int adjoint_a(const float* __restrict__ M, const float* q_in, const float*
       const Q4T<Dual> rot = lift4<Dual>(Q4{q[qa + 3], q[qa + 4], q[qa + 5], q[qa + 6]});
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        const V3T<Dual> ang{dual(qd1[da + 3], k == 0 ? 1.f : 0.f),
-                            dual(qd1[da + 4], k == 1 ? 1.f : 0.f),
-                            dual(qd1[da + 5], k == 2 ? 1.f : 0.f)};
+        const V3T<Dual> ang{dual(rec[R::QD1 + da + 3], k == 0 ? 1.f : 0.f),
+                            dual(rec[R::QD1 + da + 4], k == 1 ? 1.f : 0.f),
+                            dual(rec[R::QD1 + da + 5], k == 2 ? 1.f : 0.f)};
         const Q4T<Dual> r1 = integrate_rot(rot, ang, dt);
         gqdt[da + 3 + k] += gq1[qa + 3] * r1.w.d + gq1[qa + 4] * r1.x.d +
                             gq1[qa + 5] * r1.y.d + gq1[qa + 6] * r1.z.d;
@@ -354,83 +185,83 @@ BGX_D int adjoint_a(const float* __restrict__ M, const float* q_in, const float*
         if ((T::DANC[i] >> j) & 1u) Mb[mpat_idx<T>(i, j)] -= a[i] * c[j] + a[j] * c[i];
     }
   };
-  // Jbar += a c^T on the contact rows (kept in the record)
+  // Jbar += a c^T on the active contact rows (in the record)
+#pragma unroll 1
+  for (int r = 0; r < 4 * n_c; ++r)
 #pragma unroll
-  for (int s = 0; s < KS; ++s)
-    if (s < n_c)
-#pragma unroll
-      for (int k = 0; k < 4 * NV; ++k) rec[R::JB + 4 * s * NV + k] = 0.f;
+    for (int d = 0; d < NV; ++d) rec[R::JB + r * NV + d] = 0.f;
   auto jbar_add = [&](const float* a, const float* c) {
+#pragma unroll 1
+    for (int r = 0; r < 4 * n_c; ++r) {
+      const float ar = a[r];
 #pragma unroll
-    for (int s = 0; s < KS; ++s)
-      if (s < n_c)
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const float ar = a[4 * s + k];
-#pragma unroll
-          for (int d = 0; d < NV; ++d) rec[R::JB + (4 * s + k) * NV + d] += ar * c[d];
-        }
+      for (int d = 0; d < NV; ++d) rec[R::JB + r * NV + d] += ar * c[d];
+    }
   };
-  float Db[NR];
-#pragma unroll
-  for (int r = 0; r < NR; ++r) Db[r] = 0.f;
+  float Db[NR], xb[NR], yb[NR], bb[NR];
+#pragma unroll 1
+  for (int r = 0; r < NR; ++r) Db[r] = xb[r] = yb[r] = bb[r] = 0.f;
 
-  // qdd = u + w: u-bar = qdd-bar; w = M^-1 J^T x
-  float ub[NV], z[NV];
+  // ---- qdd = u + w, w = M^-1 J^T x
+  float ub[NV], z[NV], u[NV], w[NV];
 #pragma unroll
-  for (int d = 0; d < NV; ++d) ub[d] = z[d] = dt * gqdt[d];
-  float xb[NR], yb[NR], bb[NR];
-#pragma unroll
-  for (int r = 0; r < NR; ++r) xb[r] = yb[r] = bb[r] = 0.f;
-  if (has_rows) {
+  for (int d = 0; d < NV; ++d) {
+    ub[d] = z[d] = dt * gqdt[d];
+    u[d] = rec[R::U + d];
+    w[d] = rec[R::W + d];
+  }
+  if (has_rows && iters > 0) {
+    float x[NR];
+    row(R::XS, iters - 1, x);
     ltl_solve<T>(t, z);
     mbar_sub(z, w);
     J_apply(z, xb);
-    jbar_add(X, z);
+    jbar_add(x, z);
   }
-  // unrolled FISTA, last iteration first
+  // ---- unrolled FISTA, last iteration first:
+  //   r_k = A y_k + b, g_k = A r_k, x_{k+1} = relu(y_k - eta_k g_k),
+  //   y_{k+1} = x_{k+1} + beta_k (x_{k+1} - x_k)
+#pragma unroll 1
   for (int k = iters - 1; k >= 0; --k) {
-    const float bk = betas[k], ek = etas[k];
-    float wb[NR], gb[NR], rb[NR], ay[NR];
-#pragma unroll
+    const float bk = rec[R::BETA + k], ek = rec[R::ETA + k];
+    float wb[NR], gb[NR], rb[NR], ay[NR], xk1[NR], rk[NR];
+    row(R::XS, k, xk1);
+    row(R::RS, k, rk);
+#pragma unroll 1
     for (int r = 0; r < NR; ++r) {
       const float xt = xb[r] + (1.f + bk) * yb[r];
       xb[r] = -bk * yb[r];
-      wb[r] = Xs[k][r] > 0.f ? xt : 0.f;
+      wb[r] = xk1[r] > 0.f ? xt : 0.f;
       gb[r] = -ek * wb[r];
     }
     float ug[NV], ur[NV], urb[NV];
     A_apply(gb, rb, ug);  // r-bar = A g-bar
-    float rk[NR];
-#pragma unroll
-    for (int r = 0; r < NR; ++r) rk[r] = Rs[k][r];
     JT_apply(rk, ur);
     ltl_solve<T>(t, ur);
     jbar_add(gb, ur);
     jbar_add(rk, ug);
     mbar_sub(ug, ur);
-#pragma unroll
+#pragma unroll 1
     for (int r = 0; r < NR; ++r) Db[r] += gb[r] * rk[r];
     A_apply(rb, ay, urb);  // y-bar = w-bar + A r-bar
-    if (k > 0) {
+    if (k > 0) {  // y_0 = 0
       float yk[NR], uy[NV];
-#pragma unroll
-      for (int r = 0; r < NR; ++r) yk[r] = Ys[k][r];
+      row(R::YS, k, yk);
       JT_apply(yk, uy);
       ltl_solve<T>(t, uy);
       jbar_add(rb, uy);
       jbar_add(yk, urb);
       mbar_sub(urb, uy);
-#pragma unroll
+#pragma unroll 1
       for (int r = 0; r < NR; ++r) Db[r] += rb[r] * yk[r];
     }
-#pragma unroll
+#pragma unroll 1
     for (int r = 0; r < NR; ++r) {
       yb[r] = wb[r] + ay[r];
       bb[r] += rb[r];
     }
   }
-  // b = J u - aref
+  // ---- b = J u - aref
   {
     float jtb[NV];
     JT_apply(bb, jtb);
@@ -438,13 +269,13 @@ BGX_D int adjoint_a(const float* __restrict__ M, const float* q_in, const float*
     for (int d = 0; d < NV; ++d) ub[d] += jtb[d];
     jbar_add(bb, u);
   }
-  // u = M^-1 qf
+  // ---- u = M^-1 qf
   float qfb[NV];
 #pragma unroll
   for (int d = 0; d < NV; ++d) qfb[d] = ub[d];
   ltl_solve<T>(t, qfb);
   mbar_sub(qfb, u);
-  // actuators: qf[da] += clip(gain clip(act) + bias, force) gear
+  // ---- actuators: qf[da] += clip(gain clip(act) + bias, force) gear
 #pragma unroll
   for (int a = 0; a < T::NU; ++a) {
     const float* ac = M + T::OFF_ACT + a * ACT_STRIDE;
@@ -461,13 +292,12 @@ BGX_D int adjoint_a(const float* __restrict__ M, const float* q_in, const float*
   for (int d = 0; d < NV; ++d) rec[R::QFB + d] = qfb[d];
 #pragma unroll
   for (int k = 0; k < R::NMB; ++k) rec[R::MB + k] = Mb[k];
-#pragma unroll
+#pragma unroll 1
   for (int r = 0; r < NR; ++r) rec[R::DB + r] = Db[r], rec[R::AB + r] = -bb[r];
 #pragma unroll
-  for (int k = 0; k < NQ; ++k) rec[R::GQ1 + k] = gq1[k];
+  for (int k = 0; k < T::NQ; ++k) rec[R::GQ1 + k] = gq1[k];
 #pragma unroll
-  for (int d = 0; d < NV; ++d) rec[R::QD1 + d] = qd1[d], rec[R::GQDT + d] = gqdt[d];
-  return n_c > T::KC ? ST_SLOW_PATH : 0;
+  for (int d = 0; d < NV; ++d) rec[R::GQDT + d] = gqdt[d];
 }
 
 // ---------------------------------------------------------------------------

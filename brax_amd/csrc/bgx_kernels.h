@@ -152,9 +152,28 @@ __global__ __launch_bounds__(WAVE) void k_rollout(Launch a) {
   if (a.status) a.status[env] = a.status_or ? (a.status[env] | st) : st;
 }
 
-// adjoint stage A: one thread per env
+// adjoint stage A1: the substep with a slot per candidate, recording its tape
 template <class T>
-__global__ __launch_bounds__(WAVE) void k_adj_a(AdjLaunch a) {
+__global__ __launch_bounds__(WAVE) void k_adj_fwd(AdjLaunch a) {
+  __shared__ float smem[lds_floats<T>() * WAVE];
+  constexpr int NQ = T::NQ, NV = T::NV, KS = TapeLayout<T>::KS;
+  const int env = blockIdx.x * WAVE + threadIdx.x;
+  if (env >= a.n) return;
+  float q[NQ], qd[NV], act[T::NU > 0 ? T::NU : 1];
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) q[k] = a.q[(long long)env * NQ + k];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) qd[k] = a.qd[(long long)env * NV + k];
+#pragma unroll
+  for (int k = 0; k < T::NU; ++k) act[k] = a.act ? a.act[(long long)env * T::NU + k] : 0.f;
+  PrivRows<KS> rows;
+  substep_ks<T, KS>(a.M, q, qd, act, smem + threadIdx.x, rows, nullptr,
+                    RecPtr{a.rec + env, a.n});
+}
+
+// adjoint stage A2: the reverse sweep from the tape
+template <class T>
+__global__ __launch_bounds__(WAVE) void k_adj_rev(AdjLaunch a) {
   constexpr int NQ = T::NQ, NV = T::NV;
   const int env = blockIdx.x * WAVE + threadIdx.x;
   if (env >= a.n) return;
@@ -176,12 +195,7 @@ __global__ __launch_bounds__(WAVE) void k_adj_a(AdjLaunch a) {
     act[k] = a.act ? a.act[(long long)env * T::NU + k] : 0.f;
     gact[k] = 0.f;
   }
-  // the return value (slow-path bit) is not used here: the recompute launch
-  // reports the status.  With a status update in this kernel, hipcc's gfx950
-  // code for the ant topology returned wrong cotangents in some lanes, while
-  // the same source built for the host is clean under address, undefined
-  // and memory sanitizers and matches autograd.
-  adjoint_a<T>(a.M, q, qd, act, gq1, gqd1, RecPtr{a.rec + env, a.n}, gact);
+  adjoint_rev<T>(a.M, q, qd, act, gq1, gqd1, RecPtr{a.rec + env, a.n}, gact);
 #pragma unroll
   for (int k = 0; k < T::NU; ++k) a.gact[(long long)env * T::NU + k] += gact[k];
 }
@@ -211,7 +225,8 @@ struct TopoOps {
   int per_env;  // one model table per env
   hipError_t (*init)(const Launch&, hipStream_t);
   hipError_t (*rollout)(const Launch&, hipStream_t);
-  hipError_t (*adj_a)(const AdjLaunch&, hipStream_t);
+  hipError_t (*adj_fwd)(const AdjLaunch&, hipStream_t);
+  hipError_t (*adj_rev)(const AdjLaunch&, hipStream_t);
   hipError_t (*adj_b)(const AdjLaunch&, hipStream_t);
 };
 
@@ -225,8 +240,12 @@ struct Ops {
     hipLaunchKernelGGL(k_rollout<T>, dim3((a.n + WAVE - 1) / WAVE), dim3(WAVE), 0, s, a);
     return hipGetLastError();
   }
-  static hipError_t adj_a(const AdjLaunch& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_adj_a<T>, dim3((a.n + WAVE - 1) / WAVE), dim3(WAVE), 0, s, a);
+  static hipError_t adj_fwd(const AdjLaunch& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_adj_fwd<T>, dim3((a.n + WAVE - 1) / WAVE), dim3(WAVE), 0, s, a);
+    return hipGetLastError();
+  }
+  static hipError_t adj_rev(const AdjLaunch& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_adj_rev<T>, dim3((a.n + WAVE - 1) / WAVE), dim3(WAVE), 0, s, a);
     return hipGetLastError();
   }
   static hipError_t adj_b(const AdjLaunch& a, hipStream_t s) {
@@ -244,9 +263,10 @@ struct Ops {
               &init,
               &rollout,
               nullptr,
+              nullptr,
               nullptr};
     // the adjoint is compiled for topologies with a shared model table
-    if constexpr (!T::PER_ENV) t.adj_a = &adj_a, t.adj_b = &adj_b;
+    if constexpr (!T::PER_ENV) t.adj_fwd = &adj_fwd, t.adj_rev = &adj_rev, t.adj_b = &adj_b;
     return t;
   }
 };

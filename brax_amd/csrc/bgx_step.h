@@ -797,14 +797,83 @@ struct PrivRows {
 };
 
 // ---------------------------------------------------------------------------
+// optional record of a substep's intermediates (the adjoint's tape): with a
+// slot per candidate, the tape holds what the reverse sweep needs -- the
+// tree (cdof, LTL factor), the contact slots, the unconstrained and
+// constraint accelerations and every projected-gradient iterate.  The step
+// kernels pass NoTape, which compiles to nothing.
+
+constexpr int ADJ_MAXITER = 8;  // most solver iterations the tape holds
+
+template <class T>
+struct TapeLayout {
+  static constexpr int NV = T::NV, NQ = T::NQ;
+  static constexpr int KS = T::NCAND > 0 ? T::NCAND : 1;  // one slot per candidate
+  static constexpr int NRC = 4 * KS;                      // contact rows
+  static constexpr int NLR = T::NLIM > 0 ? T::NLIM : 1;
+  static constexpr int NR = NRC + NLR;                    // contact rows, then limit rows
+  static constexpr int NMB = mpat_row<T>(T::NV);
+  static constexpr int CSLOT = 0;              // [KS] slot of each candidate, -1 inactive
+  static constexpr int CFB = CSLOT + KS;       // [KS] capsule frame fallback taken
+  static constexpr int NC = CFB + KS;          // active contacts
+  static constexpr int ITERS = NC + 1;         // solver iterations run
+  static constexpr int CA = ITERS + 1;         // [3 NV] cdof angular
+  static constexpr int CV = CA + 3 * NV;       // [3 NV] cdof linear
+  static constexpr int LM = CV + 3 * NV;       // [NMB] LTL factor (pattern order)
+  static constexpr int INV = LM + NMB;         // [NV] 1 / L_kk
+  static constexpr int SLINK = INV + NV;       // [KS] link of each slot
+  static constexpr int SR = SLINK + KS;        // [3 KS] lever arm
+  static constexpr int SD0 = SR + 3 * KS;      // [3 KS] normal
+  static constexpr int SD1 = SD0 + 3 * KS;     // [3 KS] mu t1
+  static constexpr int SD2 = SD1 + 3 * KS;     // [3 KS] mu t2
+  static constexpr int SDIAG = SD2 + 3 * KS;   // [KS] regularizer of the slot's rows
+  static constexpr int LSIDE = SDIAG + KS;     // [NLR] limit row sign
+  static constexpr int LDIAG = LSIDE + NLR;    // [NLR] limit row regularizer
+  static constexpr int U = LDIAG + NLR;        // [NV] M^-1 qf_smooth
+  static constexpr int W = U + NV;             // [NV] M^-1 J^T x
+  static constexpr int QD1 = W + NV;           // [NV] qd after the substep
+  static constexpr int ETA = QD1 + NV;         // [ADJ_MAXITER] accepted step sizes
+  static constexpr int BETA = ETA + ADJ_MAXITER;      // [ADJ_MAXITER] momentum factors
+  static constexpr int YS = BETA + ADJ_MAXITER;       // [ADJ_MAXITER][NR] y_k
+  static constexpr int RS = YS + ADJ_MAXITER * NR;    // [ADJ_MAXITER][NR] A y_k + b
+  static constexpr int XS = RS + ADJ_MAXITER * NR;    // [ADJ_MAXITER][NR] x_{k+1}
+  static constexpr int SIZE = XS + ADJ_MAXITER * NR;
+};
+
+struct NoTape {
+  static constexpr bool ON = false;
+};
+
+// tape in global memory, field-major: field f of this env at p[f * n]
+struct RecPtr {
+  static constexpr bool ON = true;
+  float* p;
+  long long n;
+  BGX_D float& operator[](int f) const { return p[(long long)f * n]; }
+};
+
+// pyramid directions of a slot in the generic form (n, mu t1, mu t2)
+BGX_D void slot_dirs(const SlotGeo<true>& g, V3& d0, V3& d1, V3& d2) {
+  d0 = v3(0.f, 0.f, 1.f), d1 = v3(g.ax, g.ay, 0.f), d2 = v3(-g.ay, g.ax, 0.f);
+}
+BGX_D void slot_dirs(const SlotGeo<false>& g, V3& d0, V3& d1, V3& d2) {
+  d0 = g.d0, d1 = g.d1, d2 = g.d2;
+}
+template <class Tp>
+BGX_D void tape_v3(const Tp& tape, int f, V3 v) {
+  tape[f] = v.x, tape[f + 1] = v.y, tape[f + 2] = v.z;
+}
+
+// ---------------------------------------------------------------------------
 // one substep: pipeline.step (pipeline.py:64-94) with KS contact slots.
 // xp, xr: link frames at (q, qd) (kinematics.forward); lds: this lane's
 // link scratch (6 floats per link) followed by space for 54 more floats
 
-template <class T, int KS, class Rows>
+template <class T, int KS, class Rows, class Tape = NoTape>
 BGX_D bool substep_impl(const float* __restrict__ M, float* q, float* qd,
                         const float* act, const V3* xp, const Q4* xr, V3 rc0,
-                        float* lds, Rows& rows, float* dbg) {
+                        float* lds, Rows& rows, float* dbg, Tape tape = Tape{}) {
+  using TL = TapeLayout<T>;
   constexpr int NV = T::NV, L = T::L;
   constexpr int NL = T::NLIM > 0 ? T::NLIM : 1;
   constexpr bool PZ = T::PLANE_Z != 0;
@@ -823,11 +892,13 @@ BGX_D bool substep_impl(const float* __restrict__ M, float* q, float* qd,
     V3 ctr, axis;
     candidate_center<T>(M, c, xp, xr, ctr, axis);
     const float dist = candidate_dist<T>(M, c, ctr);
+    if constexpr (Tape::ON) tape[TL::CSLOT + c] = -1.f, tape[TL::CFB + c] = 0.f;
     if (dist < 0.f) {
       const int l = T::CAND_LINK[c];
       const float* cd = cand_rec<T>(M, c);
       V3 pos, t1, t2;
-      contact_frame<T>(M, c, ctr, axis, dist, -1, pos, t1, t2);
+      const bool fb = contact_frame<T>(M, c, ctr, axis, dist, -1, pos, t1, t2);
+      if constexpr (Tape::ON) tape[TL::CSLOT + c] = (float)n_c, tape[TL::CFB + c] = fb ? 1.f : 0.f;
       const V3 n = PZ ? v3(0.f, 0.f, 1.f) : ldv3(cd + CD_PN);
       const float mu = cd[CD_MU];
       SlotGeo<PZ> g;
@@ -854,6 +925,21 @@ BGX_D bool substep_impl(const float* __restrict__ M, float* q, float* qd,
   // more active contacts than slots: leave the state untouched and let the
   // caller take the path with a slot per candidate
   if (n_c > KS) return false;
+  if constexpr (Tape::ON) {
+    tape[TL::NC] = (float)n_c;
+#pragma unroll
+    for (int s = 0; s < KS; ++s)
+      if (s < n_c) {
+        V3 d0, d1, d2;
+        slot_dirs(sg[s], d0, d1, d2);
+        tape[TL::SLINK + s] = (float)sg[s].link;
+        tape_v3(tape, TL::SR + 3 * s, sg[s].r);
+        tape_v3(tape, TL::SD0 + 3 * s, d0);
+        tape_v3(tape, TL::SD1 + 3 * s, d1);
+        tape_v3(tape, TL::SD2 + 3 * s, d2);
+        tape[TL::SDIAG + s] = sl_diag[s];
+      }
+  }
 
   // ---- dynamics.transform_com (dynamics.py:27-89): cinr, cdof
   V3 cc[L];
@@ -897,6 +983,19 @@ BGX_D bool substep_impl(const float* __restrict__ M, float* q, float* qd,
 #pragma unroll
   for (int d = 0; d < NV; ++d) u[d] = qf[d];
   ltl_solve<T>(t, u);
+  if constexpr (Tape::ON) {
+#pragma unroll
+    for (int d = 0; d < NV; ++d) {
+      tape_v3(tape, TL::CA + 3 * d, t.ca[d]);
+      tape_v3(tape, TL::CV + 3 * d, t.cv[d]);
+      tape[TL::INV + d] = t.inv[d];
+      tape[TL::U + d] = u[d];
+      tape[TL::LM + mpat_idx<T>(d, d)] = t.lm(d, d);
+#pragma unroll
+      for (int j = 0; j < d; ++j)
+        if ((T::DANC[d] >> j) & 1u) tape[TL::LM + mpat_idx<T>(d, j)] = t.lm(d, j);
+    }
+  }
 
   // ---- contact rows: b = J u - aref, aref = -b J qd - k imp dist
   // (constraint.py:58-65, :205-210); link velocities from lds (u) and lds2 (qd)
@@ -956,6 +1055,7 @@ BGX_D bool substep_impl(const float* __restrict__ M, float* q, float* qd,
     const float aref = act ? (-im[IM_B] * vel - im[IM_K] * imp * pos) : 0.f;
     lside[i] = side;
     ldiag[i] = act ? df[DF_INVWEIGHT] * imp_ratio(imp) : 0.f;
+    if constexpr (Tape::ON) tape[TL::LSIDE + i] = side, tape[TL::LDIAG + i] = ldiag[i];
     lb[i] = side * u[d] - aref;
     any_lim = any_lim || act;
   }
@@ -1044,7 +1144,9 @@ BGX_D bool substep_impl(const float* __restrict__ M, float* q, float* qd,
     for (int k = 0; k < 4; ++k) lv.at(k, i) = 0.f;
 
   const bool has_rows = (n_c > 0) || any_lim;
-  const int maxiter = (int)M[H_MAXITER], maxls = (int)M[H_MAXLS];
+  const int maxiter = Tape::ON && (int)M[H_MAXITER] > ADJ_MAXITER
+                          ? ADJ_MAXITER : (int)M[H_MAXITER];
+  const int maxls = (int)M[H_MAXLS];
   const float tol = M[H_TOL], lseps = M[H_LSEPS];
   float tk = 1.f, eta = 1.f, err = INFINITY;
   bool running = has_rows;
@@ -1070,6 +1172,22 @@ BGX_D bool substep_impl(const float* __restrict__ M, float* q, float* qd,
             [&](int i) { return lv.at(3, i); },
             [&](int s, int k, float o) { rows.at(RV_G, s, k) = o; },
             [&](int i, float o) { lv.at(2, i) = o; });
+    if constexpr (Tape::ON) {  // y_k and r_k = A y_k + b
+      const int o = it * TL::NR;
+#pragma unroll
+      for (int s = 0; s < KS; ++s)
+        if (s < n_c)
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            tape[TL::YS + o + 4 * s + k] = rows.at(RV_Y, s, k);
+            tape[TL::RS + o + 4 * s + k] = rows.at(RV_R, s, k);
+          }
+#pragma unroll
+      for (int i = 0; i < T::NLIM; ++i) {
+        tape[TL::YS + o + TL::NRC + i] = lv.at(1, i);
+        tape[TL::RS + o + TL::NRC + i] = lv.at(3, i);
+      }
+    }
     // line search: trial x+ = relu(y - eta g), residual into r
     bool cond = true;
     for (int ls = 0; ls <= maxls && cond; ++ls) {
@@ -1126,13 +1244,16 @@ BGX_D bool substep_impl(const float* __restrict__ M, float* q, float* qd,
           const float xp = fmaxf(rows.at(RV_Y, s, k) - e_ * rows.at(RV_G, s, k), 0.f);
           rows.at(RV_Y, s, k) = xp + beta * (xp - rows.at(RV_X, s, k));
           rows.at(RV_X, s, k) = xp;
+          if constexpr (Tape::ON) tape[TL::XS + it * TL::NR + 4 * s + k] = xp;
         }
 #pragma unroll
     for (int i = 0; i < T::NLIM; ++i) {
       const float xp = fmaxf(lv.at(1, i) - e_ * lv.at(2, i), 0.f);
       lv.at(1, i) = xp + beta * (xp - lv.at(0, i));
       lv.at(0, i) = xp;
+      if constexpr (Tape::ON) tape[TL::XS + it * TL::NR + TL::NRC + i] = xp;
     }
+    if constexpr (Tape::ON) tape[TL::ETA + it] = e_, tape[TL::BETA + it] = beta;
     eta = eta <= 1e-6f ? 1.f : eta * 2.f;
     tk = tn;
     iters += 1;
@@ -1140,15 +1261,22 @@ BGX_D bool substep_impl(const float* __restrict__ M, float* q, float* qd,
 
   // ---- qdd = M^-1 (qf_smooth + J^T x) (integrator.py:58-64)
   float qdd[NV];
+  if constexpr (Tape::ON) tape[TL::ITERS] = (float)iters;
   if (has_rows) {
     apply_jt([&](int s, int k) { return rows.at(RV_X, s, k); },
              [&](int i) { return lv.at(0, i); }, qdd);
     ltl_solve<T>(t, qdd);
 #pragma unroll
-    for (int d = 0; d < NV; ++d) qdd[d] = u[d] + qdd[d];
+    for (int d = 0; d < NV; ++d) {
+      if constexpr (Tape::ON) tape[TL::W + d] = qdd[d];
+      qdd[d] = u[d] + qdd[d];
+    }
   } else {
 #pragma unroll
-    for (int d = 0; d < NV; ++d) qdd[d] = u[d];
+    for (int d = 0; d < NV; ++d) {
+      if constexpr (Tape::ON) tape[TL::W + d] = 0.f;
+      qdd[d] = u[d];
+    }
   }
   if (dbg) {  // per-env probe of the last substep: qdd, counts
 #pragma unroll
@@ -1162,21 +1290,26 @@ BGX_D bool substep_impl(const float* __restrict__ M, float* q, float* qd,
   const float dt = M[H_DT];
 #pragma unroll
   for (int d = 0; d < NV; ++d) qd[d] = qd[d] + qdd[d] * dt;
+  if constexpr (Tape::ON) {
+#pragma unroll
+    for (int d = 0; d < NV; ++d) tape[TL::QD1 + d] = qd[d];
+  }
   integrate_q<T>(q, qd, dt);
   return true;
 }
 
 // kinematics, root COM, then substep_impl with KS slots
-template <class T, int KS, class Rows>
+template <class T, int KS, class Rows, class Tape = NoTape>
 BGX_D bool substep_ks(const float* __restrict__ M, float* q, float* qd,
-                      const float* act, float* lds, Rows& rows, float* dbg) {
+                      const float* act, float* lds, Rows& rows, float* dbg,
+                      Tape tape = Tape{}) {
   static_assert(T::L > 0, "empty topology");
   static_assert(T::ROOT[T::L - 1] == 0, "one kinematic tree per topology");
   V3 xp[T::L];
   Q4 xr[T::L];
   forward_kinematics<T, false, float>(M, q, qd, xp, xr, nullptr, nullptr);
   const V3 rc = root_com<T>(M, xp, xr);  // dynamics.py:37-43
-  return substep_impl<T, KS>(M, q, qd, act, xp, xr, rc, lds, rows, dbg);
+  return substep_impl<T, KS>(M, q, qd, act, xp, xr, rc, lds, rows, dbg, tape);
 }
 
 // an opaque copy of the model pointer (offset by the table's zero slot, which

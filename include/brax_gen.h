@@ -131,10 +131,7 @@ int64_t bgx_adjoint_workspace_bytes(const bgx_model* model, int32_t n_envs,
  *     receives BGX_STATUS_* bits of the recomputed substeps.
  * workspace: device memory of bgx_adjoint_workspace_bytes() bytes.  The
  * call enqueues kernels only (no allocation, no synchronization).
- * Switched off in this build: returns BGX_ERR_DISABLED after validating
- * its arguments (its gfx950 kernels gave wrong cotangents for some
- * topologies and a memory fault in a test build; the same source built for
- * the host matches autograd). */
+ * Topologies with per-env model tables return BGX_ERR_TOPOLOGY. */
 int bgx_rollout_adjoint(const bgx_model* model, int32_t n_envs, int32_t n_steps,
                         int32_t n_frames, int32_t ckpt_every,
                         const float* ckpt_q, const float* ckpt_qd,

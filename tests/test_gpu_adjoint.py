@@ -21,12 +21,7 @@ from brax_amd import systems as S
 from brax_amd.generalized import pipeline
 from oracle.generalized import Oracle
 
-pytestmark = [
-    pytest.mark.gpu,
-    pytest.mark.skip(reason="bgx_rollout_adjoint is switched off: its gfx950 stage-A kernel "
-                            "returned wrong cotangents for some topologies and faulted in a "
-                            "test build; the host build of the same source matches autograd"),
-]
+pytestmark = pytest.mark.gpu
 DEV = torch.device("cuda", 0)
 GRAD_RTOL = 1e-3
 
