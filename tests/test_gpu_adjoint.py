@@ -187,3 +187,68 @@ def test_adjoint_ignores_workspace_contents():
   for x, y in zip(*outs):
     assert np.isfinite(y).all()
     np.testing.assert_array_equal(x, y)
+
+
+def test_adjoint_vs_float64_central_differences():
+  """The analytic-policy-gradient loss of the differentiable-rollout workload
+  (-sum_t v_torso,x + 1e-3 sum |act|^2) on 16 quadrupeds over 8 env steps:
+  the GPU adjoint's gradient at 64 sampled coordinates (actions and initial
+  qd) against float64 central differences of the float64 oracle.  The loss
+  is piecewise smooth: line-search halvings, iteration counts and contact
+  activations switch discretely, and a perturbation that crosses such an
+  event changes the loss by a jump no derivative sees.  With eps = 1e-6 such
+  crossings are rare: at least 90% of the coordinates must agree to 1e-3
+  relative (of the larger of |fd| and 1% of the largest |fd|).  The eps =
+  1e-3 differences are reported."""
+  sys = S.quadruped(0)
+  n, T = 16, 8
+  q0, qd0, acts, _, _ = _inputs(sys, n, T, seed=11)
+  q0t = torch.tensor(q0, device=DEV)
+  qd0t = torch.tensor(qd0, device=DEV, requires_grad=True)
+  at = torch.tensor(acts, device=DEV, requires_grad=True)
+  qt, qdt, _ = pipeline.rollout_grad(sys, q0t, qd0t, at, 5, 4)
+  loss = -qdt[:, :, 0].sum() + 1e-3 * (at * at).sum()
+  loss.backward()
+  torch.cuda.synchronize()
+  ga, gqd = at.grad.cpu().numpy(), qd0t.grad.cpu().numpy()
+  rng = np.random.default_rng(5)
+  coords = []
+  for k in range(64):
+    e = int(rng.integers(n))
+    if k % 4 == 0:
+      coords.append(("qd", e, int(rng.integers(sys.qd_size()))))
+    else:
+      coords.append(("act", e, (int(rng.integers(T)), int(rng.integers(sys.act_size())))))
+  g = np.array([gqd[e, idx] if kind == "qd" else ga[idx[0], e, idx[1]]
+                for kind, e, idx in coords])
+
+  def central_differences(eps):
+    # one float64 oracle batch: each coordinate's env perturbed by +eps, -eps
+    qb, qdb, ab = [], [], []
+    for kind, e, idx in coords:
+      for sgn in (1.0, -1.0):
+        qdx, ax = qd0[e].astype(np.float64).copy(), acts[:, e].astype(np.float64).copy()
+        if kind == "qd":
+          qdx[idx] += sgn * eps
+        else:
+          ax[idx] += sgn * eps
+        qb.append(q0[e].astype(np.float64))
+        qdb.append(qdx)
+        ab.append(ax)
+    o = Oracle(sys, torch.float64, inverse="exact")
+    st = o.init(np.stack(qb), np.stack(qdb))
+    ab = np.stack(ab, 1)  # (T, 2 * 64, nu)
+    lossb = 1e-3 * (ab * ab).sum(axis=(0, 2))
+    for t in range(T):
+      st = o.env_step(st, ab[t], 5)
+      lossb = lossb - st.qd[:, 0].numpy()
+    return (lossb[0::2] - lossb[1::2]) / (2 * eps)
+
+  stats = {}
+  for eps in (1e-6, 1e-3):
+    fd = central_differences(eps)
+    rel = np.abs(g - fd) / np.maximum(np.abs(fd), 1e-2 * np.abs(fd).max())
+    stats[eps] = rel
+    print(f"\nFD eps {eps:g}: median rel err {np.median(rel):.2e}, "
+          f"within 1e-3: {(rel < 1e-3).mean():.2f}", flush=True)
+  assert (stats[1e-6] < 1e-3).mean() >= 0.9
