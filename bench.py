@@ -143,6 +143,10 @@ def forward_bench(args, name):
   acts.uniform_(-act_range, act_range, generator=g)
   q_init = torch.tensor(q0, device=dev)
   qd_init = torch.tensor(qd0, device=dev)
+  epw = 64
+  if not args.no_tune:
+    from brax_amd.generalized import autotune
+    epw = autotune.tune_step(sys_, q_init, qd_init, acts[:2], N_FRAMES)
   q = q_init.clone()
   qd = qd_init.clone()
   status = torch.zeros(n, dtype=torch.int32, device=dev)
@@ -225,7 +229,8 @@ def forward_bench(args, name):
                              f"{N_FRAMES} substeps x dt {sys_.opt.timestep:g} per env "
                              "step, one HIP-graph rollout launch",
                  "envs": n, "env_steps": K, "substeps_per_env_step": N_FRAMES,
-                 "qd_size": nv, "parallelism": "none (1 GPU)"},
+                 "qd_size": nv, "envs_per_workgroup": epw,
+                 "parallelism": "none (1 GPU)"},
       "roofline": {"bound": "valu", "achieved": achieved,
                    "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                    "frac": achieved / FP32_PEAK_TFLOPS,
@@ -267,6 +272,15 @@ def apg_bench(args):
     return pipeline.rollout(sys_, st, acts0, N_FRAMES, trajectory=True, frames=False)
 
   from brax_amd import _lib
+  epw = [64, 64]
+  if not args.no_tune:
+    from brax_amd.generalized import autotune
+    epw[0] = autotune.tune_step(sys_, q0t, qd0t, acts0[:2], N_FRAMES)
+    try:
+      epw[1] = autotune.tune_adjoint(sys_, q0t, qd0t, acts0[:2], N_FRAMES)
+    except _lib.BgxError as e:
+      print(f"apg workload unavailable: {e}", file=sys.stderr)
+      sys.exit(3)
   try:
     fwd_bwd()
   except _lib.BgxError as e:
@@ -306,6 +320,7 @@ def apg_bench(args):
                              "1e-3 sum |act|^2, gradient w.r.t. actions and qd0, "
                              f"checkpoint every {ckpt} env steps",
                  "envs": n, "horizon": T, "ckpt_every": ckpt,
+                 "envs_per_workgroup": {"step": epw[0], "adjoint": epw[1]},
                  "parallelism": "none (1 GPU)"},
       "forward_value": fwd_rate,
       "fwd_adjoint_over_forward": fb_rate / fwd_rate,
@@ -346,6 +361,9 @@ def hetero_bench(args):
   q = [x.clone() for x in q0]
   qd = [x.clone() for x in qd0]
   side = torch.cuda.Stream(device=dev)
+  epw = None
+  if not args.no_tune:
+    epw = hb.autotune(q0, qd0, acts, N_FRAMES)
 
   def reset():
     for a, b in zip(q, q0):
@@ -395,7 +413,10 @@ def hetero_bench(args):
       "data": "synthetic (seeded chain and quadruped states, U(-1,1) actions in HBM)",
       "config": {"workload": "heterogeneous batch: one HIP graph of per-topology "
                              "rollout kernels on 4 streams", "envs": n,
-                 "env_steps": K, "groups": len(hb.groups), "parallelism": "none (1 GPU)"},
+                 "env_steps": K, "groups": len(hb.groups),
+                 "envs_per_workgroup": None if epw is None else
+                 [[envs[g.envs[0]].name, g.n, e] for g, e in zip(hb.groups, epw)],
+                 "parallelism": "none (1 GPU)"},
       "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                    "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
                    "traffic": None, "kernel_ms": kernel_ms,
@@ -415,6 +436,8 @@ def main():
   ap.add_argument("--ckpt", type=int, default=16)
   ap.add_argument("--no-graph", action="store_true")
   ap.add_argument("--no-cpu", action="store_true")
+  ap.add_argument("--no-tune", action="store_true",
+                  help="64 envs per workgroup instead of the autotuner's choice")
   ap.add_argument("--calib-copy", action="store_true",
                   help="run a 256 MiB elementwise copy before the timed region "
                        "(a known-size kernel for scaling HBM counters)")

@@ -35,6 +35,8 @@ SIGNATURES = {
     "bgx_model_create": (c_int, [c_int, c_vp, c_i64, ctypes.POINTER(c_vp)]),
     "bgx_model_destroy": (c_int, [c_vp]),
     "bgx_model_topology": (c_int, [c_vp]),
+    "bgx_model_set_launch": (c_int, [c_vp, c_int, c_i32]),
+    "bgx_model_get_launch": (c_int, [c_vp, c_int]),
     "bgx_init": (c_int, [c_vp, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "bgx_step": (c_int, [c_vp, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp,
                          c_vp, c_vp, c_vp, c_vp]),

@@ -66,6 +66,7 @@ struct bgx_model {
   int64_t size;      // floats
   int maxiter;       // solver iterations (from the table header)
   int64_t n_tables;  // tables (one per env for per-env topologies)
+  int epw[2];        // envs per workgroup: step kernels, adjoint kernels (0 = 64)
 };
 
 extern "C" {
@@ -107,7 +108,7 @@ int bgx_model_create(int id, const float* host_table, int64_t n_floats, bgx_mode
     (void)hipFree(d);
     return rc;
   }
-  *out = new bgx_model{id, d, n_floats, (int)host_table[bgx::H_MAXITER], n_floats / size};
+  *out = new bgx_model{id, d, n_floats, (int)host_table[bgx::H_MAXITER], n_floats / size, {0, 0}};
   return BGX_OK;
 }
 
@@ -120,6 +121,22 @@ int bgx_model_destroy(bgx_model* m) {
 
 int bgx_model_topology(const bgx_model* m) { return m ? m->topo : -1; }
 
+int bgx_model_set_launch(bgx_model* m, int kernels, int32_t envs_per_workgroup) {
+  if (!m) return fail(BGX_ERR_ARG, "model is null");
+  if (kernels != BGX_KERNELS_STEP && kernels != BGX_KERNELS_ADJOINT)
+    return fail(BGX_ERR_ARG, "unknown kernel group");
+  const int e = envs_per_workgroup;
+  if (e != 0 && (e < 1 || e > bgx::WAVE || (e & (e - 1)) != 0))
+    return fail(BGX_ERR_ARG, "envs per workgroup must be 0 or a power of two up to 64");
+  m->epw[kernels] = e;
+  return BGX_OK;
+}
+
+int bgx_model_get_launch(const bgx_model* m, int kernels) {
+  if (!m || (kernels != BGX_KERNELS_STEP && kernels != BGX_KERNELS_ADJOINT)) return -1;
+  return m->epw[kernels] ? m->epw[kernels] : bgx::WAVE;
+}
+
 static int launch(const bgx_model* m, bool init_only, bgx::Launch a, void* stream) {
   if (!m) return fail(BGX_ERR_ARG, "model is null");
   if (!valid_topo(m->topo)) return fail(BGX_ERR_TOPOLOGY, "unknown topology id");
@@ -130,6 +147,7 @@ static int launch(const bgx_model* m, bool init_only, bgx::Launch a, void* strea
   if (kOps[m->topo]->per_env && a.n > m->n_tables)
     return fail(BGX_ERR_ARG, "more envs than per-env model tables");
   a.M = m->table;
+  a.epw = m->epw[BGX_KERNELS_STEP];
   const bgx::TopoOps* ops = kOps[m->topo];
   hipStream_t s = static_cast<hipStream_t>(stream);
   return check_hip(init_only ? ops->init(a, s) : ops->rollout(a, s), "kernel launch");
@@ -278,13 +296,14 @@ int bgx_rollout_adjoint(const bgx_model* m, int32_t n, int32_t n_steps, int32_t 
     f.act_stride = act_step_stride;
     f.sub_q = w.sub_q, f.sub_qd = w.sub_qd;
     f.status = status, f.status_or = 1;
+    f.epw = m->epw[BGX_KERNELS_ADJOINT];
     if ((rc = check_hip(ops->rollout(f, s), "recompute launch"))) return rc;
     // substeps of the segment, last first
     for (int i = ns * n_frames - 1; i >= 0; --i) {
       const int t = s0 + i / n_frames;
       const bool step_end = (i % n_frames) == n_frames - 1;
       bgx::AdjLaunch a{};
-      a.M = m->table, a.n = n;
+      a.M = m->table, a.n = n, a.epw = m->epw[BGX_KERNELS_ADJOINT];
       a.q = w.sub_q + (long long)i * n * nq;
       a.qd = w.sub_qd + (long long)i * n * nv;
       a.act = act ? act + (long long)t * act_step_stride : nullptr;

@@ -37,6 +37,7 @@ struct Launch {
   float* probe;
   float* sub_q;   // (n_steps * n_frames, n, NQ): state at the start of each substep
   float* sub_qd;  // (n_steps * n_frames, n, NV)
+  int epw;        // envs (lanes) per workgroup, 1..WAVE; 0 = WAVE
 };
 
 // one adjoint substep: state (q, qd) of substep i, its action, the running
@@ -52,7 +53,15 @@ struct AdjLaunch {
   float* adj;        // (n, NQ + NV), read by stage A, written by stage B
   float* rec;        // (SIZE, n) stage A record
   float* gact;       // (n, NU) accumulated
+  int epw;           // envs (lanes) per workgroup of A1/A2, 1..WAVE; 0 = WAVE
 };
+
+// Workgroups of fewer than WAVE lanes spread a small batch over more CUs.
+// The LDS of a workgroup keeps the WAVE-lane stride whatever its lane count,
+// so every lane computes the same instructions on the same values: results
+// do not depend on the choice.
+inline int block_lanes(int epw) { return epw > 0 && epw < WAVE ? epw : WAVE; }
+inline dim3 env_grid(long long n, int b) { return dim3((unsigned)((n + b - 1) / b)); }
 
 // the model table of env: shared, or one per env (PER_ENV topologies)
 template <class T>
@@ -87,7 +96,7 @@ BGX_D void write_frames(const Launch& a, int env, const float* q, const float* q
 
 template <class T>
 __global__ __launch_bounds__(WAVE) void k_init(Launch a) {
-  const int env = blockIdx.x * WAVE + threadIdx.x;
+  const int env = blockIdx.x * blockDim.x + threadIdx.x;
   if (env >= a.n) return;
   float q[T::NQ], qd[T::NV];
 #pragma unroll
@@ -100,7 +109,7 @@ __global__ __launch_bounds__(WAVE) void k_init(Launch a) {
 template <class T>
 __global__ __launch_bounds__(WAVE) void k_rollout(Launch a) {
   __shared__ float smem[lds_floats<T>() * WAVE];  // layout: lds_floats()
-  const int env = blockIdx.x * WAVE + threadIdx.x;
+  const int env = blockIdx.x * blockDim.x + threadIdx.x;
   if (env >= a.n) return;
   float* lds = smem + threadIdx.x;
   float q[T::NQ], qd[T::NV];
@@ -157,7 +166,7 @@ template <class T>
 __global__ __launch_bounds__(WAVE) void k_adj_fwd(AdjLaunch a) {
   __shared__ float smem[lds_floats<T>() * WAVE];
   constexpr int NQ = T::NQ, NV = T::NV, KS = TapeLayout<T>::KS;
-  const int env = blockIdx.x * WAVE + threadIdx.x;
+  const int env = blockIdx.x * blockDim.x + threadIdx.x;
   if (env >= a.n) return;
   float q[NQ], qd[NV], act[T::NU > 0 ? T::NU : 1];
 #pragma unroll
@@ -175,7 +184,7 @@ __global__ __launch_bounds__(WAVE) void k_adj_fwd(AdjLaunch a) {
 template <class T>
 __global__ __launch_bounds__(WAVE) void k_adj_rev(AdjLaunch a) {
   constexpr int NQ = T::NQ, NV = T::NV;
-  const int env = blockIdx.x * WAVE + threadIdx.x;
+  const int env = blockIdx.x * blockDim.x + threadIdx.x;
   if (env >= a.n) return;
   float q[NQ], qd[NV], gq1[NQ], gqd1[NV];
   float act[T::NU > 0 ? T::NU : 1], gact[T::NU > 0 ? T::NU : 1];
@@ -233,19 +242,23 @@ struct TopoOps {
 template <class T>
 struct Ops {
   static hipError_t init(const Launch& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_init<T>, dim3((a.n + WAVE - 1) / WAVE), dim3(WAVE), 0, s, a);
+    const int b = block_lanes(a.epw);
+    hipLaunchKernelGGL(k_init<T>, env_grid(a.n, b), dim3(b), 0, s, a);
     return hipGetLastError();
   }
   static hipError_t rollout(const Launch& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_rollout<T>, dim3((a.n + WAVE - 1) / WAVE), dim3(WAVE), 0, s, a);
+    const int b = block_lanes(a.epw);
+    hipLaunchKernelGGL(k_rollout<T>, env_grid(a.n, b), dim3(b), 0, s, a);
     return hipGetLastError();
   }
   static hipError_t adj_fwd(const AdjLaunch& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_adj_fwd<T>, dim3((a.n + WAVE - 1) / WAVE), dim3(WAVE), 0, s, a);
+    const int b = block_lanes(a.epw);
+    hipLaunchKernelGGL(k_adj_fwd<T>, env_grid(a.n, b), dim3(b), 0, s, a);
     return hipGetLastError();
   }
   static hipError_t adj_rev(const AdjLaunch& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_adj_rev<T>, dim3((a.n + WAVE - 1) / WAVE), dim3(WAVE), 0, s, a);
+    const int b = block_lanes(a.epw);
+    hipLaunchKernelGGL(k_adj_rev<T>, env_grid(a.n, b), dim3(b), 0, s, a);
     return hipGetLastError();
   }
   static hipError_t adj_b(const AdjLaunch& a, hipStream_t s) {

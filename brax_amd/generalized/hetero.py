@@ -105,6 +105,14 @@ class HeteroBatch:
     for s in streams:
       cur.wait_stream(s)
 
+  def autotune(self, q, qd, acts, n_frames: int, n_steps: int = 2):
+    """Tunes every group's envs per workgroup (autotune.tune_step) on a short
+    rollout from its current state; q, qd are not modified.  Returns the
+    choices, group by group."""
+    from brax_amd.generalized import autotune
+    return [autotune.tune_step(g.sys, q[i], qd[i], acts[i][:n_steps], n_frames)
+            for i, g in enumerate(self.groups)]
+
   def rollout(self, q, qd, acts, n_steps: int, n_frames: int):
     """Advances per-group states q[g], qd[g] (in place) by n_steps env steps;
     acts[g]: (n_steps, n_g, nu_g)."""

@@ -72,6 +72,19 @@ int bgx_model_create(int topo_id, const float* host_table, int64_t n_floats,
 int bgx_model_destroy(bgx_model* model);
 int bgx_model_topology(const bgx_model* model);
 
+/* Launch shape of a model's kernels: envs_per_workgroup lanes (1, 2, 4, ...,
+ * 64; 0 = 64) per workgroup for the step kernels (bgx_init, bgx_step,
+ * bgx_rollout, bgx_step_probe) or the adjoint kernels (bgx_rollout_adjoint,
+ * its recompute included).  Fewer lanes per workgroup spread a small batch
+ * over more compute units; results are bitwise the same for every choice.
+ * No counterpart in the reference, where XLA picks the launch shape; the
+ * autotuner (brax_amd/generalized/autotune.py) sets it. */
+#define BGX_KERNELS_STEP 0
+#define BGX_KERNELS_ADJOINT 1
+int bgx_model_set_launch(bgx_model* model, int kernels, int32_t envs_per_workgroup);
+/* the current setting (64 for 0), or -1 on bad arguments */
+int bgx_model_get_launch(const bgx_model* model, int kernels);
+
 /* pipeline.init (brax/generalized/pipeline.py:32-61): world link frames x
  * and velocities xd of N envs from q (N, NQ) and qd (N, NV).  Outputs:
  * x_pos (N, L, 3), x_rot (N, L, 4), xd_ang (N, L, 3), xd_vel (N, L, 3). */

@@ -59,6 +59,8 @@ def test_error_paths_without_gpu(lib):
   assert lib.bgx_model_create(0, table.ctypes.data, 3, ctypes.byref(out)) == 1
   assert lib.bgx_step(None, 1, 1, None, None, None, None, None, None, None,
                       None, None) == 1
+  assert lib.bgx_model_set_launch(None, 0, 16) == 1
+  assert lib.bgx_model_get_launch(None, 0) == -1
 
 
 def test_layout_constants_match_header():
