@@ -17,7 +17,11 @@
 //    ingredients -- qf_smooth, the mass matrix (a symmetric matrix on the
 //    dof-ancestor pattern), the contact rows of J, the diagonal regularizer
 //    d and aref -- plus the action gradient and the velocity pass-through of
-//    qd' = qd + dt qdd.
+//    qd' = qd + dt qdd.  The cotangent of J is never formed: J enters only
+//    through products a^T J c, and each such product is carried back to the
+//    quantities J is built from (cdof, the slots' lever arms and pyramid
+//    directions), a few floats per dof and per slot.  The row vectors of the
+//    sweep live in LDS.
 //  B (one thread per env and input coordinate): run the same formulas that
 //    build those ingredients from (q, qd) in dual numbers along one input
 //    coordinate and contract the tangents with the cotangents of stage A:
@@ -38,28 +42,71 @@ struct AdjRec : TapeLayout<T> {
   using TL = TapeLayout<T>;
   static constexpr int QFB = TL::SIZE;               // [NV]  cotangent of qf_smooth
   static constexpr int MB = QFB + TL::NV;            // [NMB] cotangent of M (symmetrized)
-  static constexpr int JB = MB + TL::NMB;            // [NRC][NV] cotangent of J contact rows
-  static constexpr int DB = JB + TL::NRC * TL::NV;   // [NR]  cotangent of d
+  static constexpr int SLB = MB + TL::NMB;           // [KS][12] cotangents of each slot's
+                                                     //   d0, d1, d2 and lever arm r
+  static constexpr int CAB = SLB + 12 * TL::KS;      // [3 NV] cotangent of cdof (angular)
+  static constexpr int CVB = CAB + 3 * TL::NV;       // [3 NV] cotangent of cdof (linear)
+  static constexpr int DB = CVB + 3 * TL::NV;        // [NR]  cotangent of d
   static constexpr int AB = DB + TL::NR;             // [NR]  cotangent of aref
   static constexpr int GQ1 = AB + TL::NR;            // [NQ]  cotangent of q'
   static constexpr int GQDT = GQ1 + TL::NQ;          // [NV]  qd' cotangent incl. the q' update
   static constexpr int SIZE = GQDT + TL::NV;
 };
 
+// LDS of the reverse sweep per lane (floats; element e of a lane at
+// base[e * lanes], lanes = rev_lanes<T>()): row vectors over the NR
+// constraint rows, each slot's geometry (link, r, d0, d1, d2, diag) and
+// cotangents (d0, d1, d2, r), and the cdof cotangents
+template <class T>
+struct RevLds {
+  using R = AdjRec<T>;
+  static constexpr int NR = R::NR, KS = R::KS, NV = T::NV;
+  static constexpr int XB = 0, YB = XB + NR, BB = YB + NR, DB = BB + NR, WB = DB + NR,
+                       GB = WB + NR, RB = GB + NR, TR = RB + NR;
+  static constexpr int SL = TR + NR;       // [KS][14]
+  static constexpr int SB = SL + 14 * KS;  // [KS][12]
+  static constexpr int CAB = SB + 12 * KS;  // [3 NV]
+  static constexpr int CVB = CAB + 3 * NV;  // [3 NV]
+  static constexpr int SIZE = CVB + 3 * NV;
+};
+
+// lanes per workgroup of the reverse sweep: the most (a power of two up to
+// WAVE) whose LDS fits in 80 KB, so two workgroups share a CU
+template <class T>
+constexpr int rev_lanes() {
+  constexpr int b = RevLds<T>::SIZE * 4;
+  return b * 64 <= 81920 ? 64 : b * 32 <= 81920 ? 32 : b * 16 <= 81920 ? 16
+       : b * 8 <= 81920 ? 8 : b * 4 <= 81920 ? 4 : 1;
+}
+
+// a vector of LDS floats with a lane stride
+template <int S>
+struct LVec {
+  float* p;
+  BGX_D float& operator[](int r) const { return p[r * S]; }
+};
+
 BGX_D V3 ld3(const RecPtr& rec, int f) { return v3(rec[f], rec[f + 1], rec[f + 2]); }
 
 // ---------------------------------------------------------------------------
 // stage A2: reverse sweep from the tape (stage A1 is substep_impl recording
-// its tape with a slot per candidate).  Slot loops are rolled (dynamic slot
-// index, tape in global memory): compact code for any candidate count.
+// its tape with a slot per candidate).  Slot and row loops are rolled over
+// the active rows only (contact rows 4 s + k of the n_c slots, then the
+// static limit rows); the row vectors and slot data sit in LDS.
 
-template <class T>
+template <class T, int AL>
 BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float* qd,
                        const float* act, const float* gq1, const float* gqd1, RecPtr rec,
-                       float* gact) {
+                       float* lds, float* gact) {
   using R = AdjRec<T>;
+  using Y = RevLds<T>;
   constexpr int NV = T::NV, L = T::L, NR = R::NR, NRC = R::NRC;
   const int n_c = (int)rec[R::NC], iters = (int)rec[R::ITERS];
+  auto vec = [&](int off) { return LVec<AL>{lds + off * AL}; };
+  const LVec<AL> xb = vec(Y::XB), yb = vec(Y::YB), bb = vec(Y::BB), Db = vec(Y::DB),
+                 wb = vec(Y::WB), gb = vec(Y::GB), rb = vec(Y::RB), tr = vec(Y::TR);
+  const LVec<AL> sl = vec(Y::SL), sb = vec(Y::SB), cab = vec(Y::CAB), cvb = vec(Y::CVB);
+  const LVec<AL> ay = gb;  // A r-bar; g-bar is dead by then
 
   Tree<T> t;
   float lm_store[T::LM_LDS ? R::NMB : 1];
@@ -83,43 +130,91 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
     any_lim = any_lim || lside[i] != 0.f;
   }
   const bool has_rows = n_c > 0 || any_lim;
+  // slot geometry into LDS, slot cotangents cleared
+#pragma unroll 1
+  for (int s = 0; s < n_c; ++s) {
+    float g[14];
+    g[0] = rec[R::SLINK + s];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      g[1 + k] = rec[R::SR + 3 * s + k];
+      g[4 + k] = rec[R::SD0 + 3 * s + k];
+      g[7 + k] = rec[R::SD1 + 3 * s + k];
+      g[10 + k] = rec[R::SD2 + 3 * s + k];
+    }
+    g[13] = rec[R::SDIAG + s];
+#pragma unroll
+    for (int k = 0; k < 14; ++k) sl[14 * s + k] = g[k];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) sb[12 * s + k] = 0.f;
+  }
+#pragma unroll
+  for (int k = 0; k < 3 * NV; ++k) cab[k] = 0.f, cvb[k] = 0.f;
+  auto slot_v3 = [&](int s, int o) {
+    return v3(sl[14 * s + o], sl[14 * s + o + 1], sl[14 * s + o + 2]);
+  };
+  // f(r) for every active row r
+  auto for_rows = [&](auto&& f) {
+#pragma unroll 1
+    for (int r = 0; r < 4 * n_c; ++r) f(r);
+#pragma unroll
+    for (int i = 0; i < T::NLIM; ++i) f(NRC + i);
+  };
+  // tape row vector of iteration k into tr (4 loads in flight per slot)
+  auto load_row = [&](int base, int k) {
+    const int b0 = base + k * NR;
+#pragma unroll 1
+    for (int s = 0; s < n_c; ++s) {
+      const float v0 = rec[b0 + 4 * s], v1 = rec[b0 + 4 * s + 1];
+      const float v2 = rec[b0 + 4 * s + 2], v3_ = rec[b0 + 4 * s + 3];
+      tr[4 * s] = v0, tr[4 * s + 1] = v1, tr[4 * s + 2] = v2, tr[4 * s + 3] = v3_;
+    }
+    float lv[R::NLR];
+#pragma unroll
+    for (int i = 0; i < T::NLIM; ++i) lv[i] = rec[b0 + NRC + i];
+#pragma unroll
+    for (int i = 0; i < T::NLIM; ++i) tr[NRC + i] = lv[i];
+  };
 
-  // row vectors: contact rows 4 s + k (s < n_c), then limit rows NRC + i
-  auto J_apply = [&](const float* z, float* out) {
+  // J z on the active rows
+  auto J_apply = [&](const float* z, const LVec<AL>& out) {
     V3 va[L], vv[L];
     link_vel<T>(t.ca, t.cv, z, va, vv);
 #pragma unroll 1
     for (int s = 0; s < n_c; ++s) {
-      const int l = (int)rec[R::SLINK + s];
+      const int l = (int)sl[14 * s];
       V3 a = va[0], v = vv[0];
 #pragma unroll
       for (int k = 1; k < L; ++k)
         if (l == k) a = va[k], v = vv[k];
-      const V3 vp = v + cross(a, ld3(rec, R::SR + 3 * s));
-      const float e0 = dot(ld3(rec, R::SD0 + 3 * s), vp);
-      const float e1 = dot(ld3(rec, R::SD1 + 3 * s), vp);
-      const float e2 = dot(ld3(rec, R::SD2 + 3 * s), vp);
+      const V3 vp = v + cross(a, slot_v3(s, 1));
+      const float e0 = dot(slot_v3(s, 4), vp);
+      const float e1 = dot(slot_v3(s, 7), vp);
+      const float e2 = dot(slot_v3(s, 10), vp);
       out[4 * s] = e0 + e1;
       out[4 * s + 1] = e0 - e1;
       out[4 * s + 2] = e0 + e2;
       out[4 * s + 3] = e0 - e2;
     }
-#pragma unroll 1
-    for (int r = 4 * n_c; r < NRC; ++r) out[r] = 0.f;
 #pragma unroll
     for (int i = 0; i < T::NLIM; ++i) out[NRC + i] = lside[i] * z[T::LIM_D[i]];
   };
-  auto JT_apply = [&](const float* x, float* tau) {
+  // slot s's pyramid rows of x as one force at the contact point
+  auto slot_force = [&](const LVec<AL>& x, int s, float& al0, float& al1, float& al2) {
+    const float x0 = x[4 * s], x1 = x[4 * s + 1], x2 = x[4 * s + 2], x3 = x[4 * s + 3];
+    al0 = (x0 + x1) + (x2 + x3), al1 = x0 - x1, al2 = x2 - x3;
+    return slot_v3(s, 4) * al0 + slot_v3(s, 7) * al1 + slot_v3(s, 10) * al2;
+  };
+  auto JT_apply = [&](const LVec<AL>& x, float* tau) {
     V3 fa[L], fv[L];
 #pragma unroll
     for (int l = 0; l < L; ++l) fa[l] = v3zero(), fv[l] = v3zero();
 #pragma unroll 1
     for (int s = 0; s < n_c; ++s) {
-      const float x0 = x[4 * s], x1 = x[4 * s + 1], x2 = x[4 * s + 2], x3 = x[4 * s + 3];
-      const V3 w = ld3(rec, R::SD0 + 3 * s) * ((x0 + x1) + (x2 + x3)) +
-                   ld3(rec, R::SD1 + 3 * s) * (x0 - x1) + ld3(rec, R::SD2 + 3 * s) * (x2 - x3);
-      const V3 ta = cross(ld3(rec, R::SR + 3 * s), w);
-      const int l = (int)rec[R::SLINK + s];
+      float a0, a1, a2;
+      const V3 w = slot_force(x, s, a0, a1, a2);
+      const V3 ta = cross(slot_v3(s, 1), w);
+      const int l = (int)sl[14 * s];
 #pragma unroll
       for (int k = 0; k < L; ++k)
         if (hosts_contact<T>(k) && l == k) fa[k] = fa[k] + ta, fv[k] = fv[k] + w;
@@ -128,20 +223,13 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
 #pragma unroll
     for (int i = 0; i < T::NLIM; ++i) tau[T::LIM_D[i]] += lside[i] * x[NRC + i];
   };
-  auto rdiag = [&](int r) {
-    return r < NRC ? (r < 4 * n_c ? rec[R::SDIAG + r / 4] : 0.f) : ldiag[r - NRC];
-  };
+  auto rdiag = [&](int r) { return r < NRC ? sl[14 * (r >> 2) + 13] : ldiag[r - NRC]; };
   // out = A x, ux = M^-1 J^T x
-  auto A_apply = [&](const float* x, float* out, float* ux) {
+  auto A_apply = [&](const LVec<AL>& x, const LVec<AL>& out, float* ux) {
     JT_apply(x, ux);
     ltl_solve<T>(t, ux);
     J_apply(ux, out);
-#pragma unroll 1
-    for (int r = 0; r < NR; ++r) out[r] = out[r] + rdiag(r) * x[r];
-  };
-  auto row = [&](int base, int k, float* out) {  // a tape row vector of iteration k
-#pragma unroll 1
-    for (int r = 0; r < NR; ++r) out[r] = rec[base + k * NR + r];
+    for_rows([&](int r) { out[r] = out[r] + rdiag(r) * x[r]; });
   };
 
   // ---- q' = f(q, qd'): cotangent of qd' from the position update
@@ -185,22 +273,61 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
         if ((T::DANC[i] >> j) & 1u) Mb[mpat_idx<T>(i, j)] -= a[i] * c[j] + a[j] * c[i];
     }
   };
-  // Jbar += a c^T on the active contact rows (in the record)
-#pragma unroll 1
-  for (int r = 0; r < 4 * n_c; ++r)
+  // Jbar += a c^T on the contact rows, carried to J's ingredients: contact
+  // row block of slot s is C E with E[i][d] = d_i . (cv_d + ca_d x r) over
+  // the dofs d above the slot's link and C = [1 1 0; 1 -1 0; 1 0 1; 1 0 -1].
+  // With alpha = C^T a_s, f = sum_i alpha_i d_i and (w, v) the velocity of
+  // the slot's link under c:  d_i-bar += alpha_i (v + w x r),
+  // r-bar += f x w,  ca_d-bar += c_d (subtree sum of r x f),
+  // cv_d-bar += c_d (subtree sum of f).
+  auto jbar_add = [&](const LVec<AL>& a, const float* c) {
+    V3 va[L], vv[L];
+    link_vel<T>(t.ca, t.cv, c, va, vv);
+    V3 fa[L], fv[L];
 #pragma unroll
-    for (int d = 0; d < NV; ++d) rec[R::JB + r * NV + d] = 0.f;
-  auto jbar_add = [&](const float* a, const float* c) {
+    for (int l = 0; l < L; ++l) fa[l] = v3zero(), fv[l] = v3zero();
 #pragma unroll 1
-    for (int r = 0; r < 4 * n_c; ++r) {
-      const float ar = a[r];
+    for (int s = 0; s < n_c; ++s) {
+      float a0, a1, a2;
+      const V3 f = slot_force(a, s, a0, a1, a2);
+      const V3 r = slot_v3(s, 1);
+      const int l = (int)sl[14 * s];
+      V3 w = va[0], v = vv[0];
 #pragma unroll
-      for (int d = 0; d < NV; ++d) rec[R::JB + r * NV + d] += ar * c[d];
+      for (int k = 1; k < L; ++k)
+        if (l == k) w = va[k], v = vv[k];
+      const V3 vp = v + cross(w, r);
+      const V3 rf = cross(f, w);
+      float* g = &sb[12 * s];
+      g[0] += a0 * vp.x, g[AL] += a0 * vp.y, g[2 * AL] += a0 * vp.z;
+      g[3 * AL] += a1 * vp.x, g[4 * AL] += a1 * vp.y, g[5 * AL] += a1 * vp.z;
+      g[6 * AL] += a2 * vp.x, g[7 * AL] += a2 * vp.y, g[8 * AL] += a2 * vp.z;
+      g[9 * AL] += rf.x, g[10 * AL] += rf.y, g[11 * AL] += rf.z;
+      const V3 ta = cross(r, f);
+#pragma unroll
+      for (int k = 0; k < L; ++k)
+        if (hosts_contact<T>(k) && l == k) fa[k] = fa[k] + ta, fv[k] = fv[k] + f;
+    }
+    V3 acc_a[L], acc_v[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) acc_a[l] = v3zero(), acc_v[l] = v3zero();
+#pragma unroll
+    for (int l = L - 1; l >= 0; --l) {
+      const V3 sa = fa[l] + acc_a[l], sv = fv[l] + acc_v[l];
+      const int nd = T::LTYPE[l] == 0 ? 6 : T::LTYPE[l];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        if (k < nd) {
+          const int d = T::DADR[l] + k;
+          cab[3 * d] += c[d] * sa.x, cab[3 * d + 1] += c[d] * sa.y, cab[3 * d + 2] += c[d] * sa.z;
+          cvb[3 * d] += c[d] * sv.x, cvb[3 * d + 1] += c[d] * sv.y, cvb[3 * d + 2] += c[d] * sv.z;
+        }
+      }
+      const int par = T::PARENT[l];
+      if (par >= 0) acc_a[par] = acc_a[par] + sa, acc_v[par] = acc_v[par] + sv;
     }
   };
-  float Db[NR], xb[NR], yb[NR], bb[NR];
-#pragma unroll 1
-  for (int r = 0; r < NR; ++r) Db[r] = xb[r] = yb[r] = bb[r] = 0.f;
+  for_rows([&](int r) { Db[r] = xb[r] = yb[r] = bb[r] = 0.f; });
 
   // ---- qdd = u + w, w = M^-1 J^T x
   float ub[NV], z[NV], u[NV], w[NV];
@@ -211,12 +338,11 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
     w[d] = rec[R::W + d];
   }
   if (has_rows && iters > 0) {
-    float x[NR];
-    row(R::XS, iters - 1, x);
+    load_row(R::XS, iters - 1);
     ltl_solve<T>(t, z);
     mbar_sub(z, w);
     J_apply(z, xb);
-    jbar_add(x, z);
+    jbar_add(tr, z);
   }
   // ---- unrolled FISTA, last iteration first:
   //   r_k = A y_k + b, g_k = A r_k, x_{k+1} = relu(y_k - eta_k g_k),
@@ -224,45 +350,41 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
 #pragma unroll 1
   for (int k = iters - 1; k >= 0; --k) {
     const float bk = rec[R::BETA + k], ek = rec[R::ETA + k];
-    float wb[NR], gb[NR], rb[NR], ay[NR], xk1[NR], rk[NR];
-    row(R::XS, k, xk1);
-    row(R::RS, k, rk);
-#pragma unroll 1
-    for (int r = 0; r < NR; ++r) {
+    load_row(R::XS, k);  // x_{k+1}
+    for_rows([&](int r) {
       const float xt = xb[r] + (1.f + bk) * yb[r];
       xb[r] = -bk * yb[r];
-      wb[r] = xk1[r] > 0.f ? xt : 0.f;
-      gb[r] = -ek * wb[r];
-    }
+      const float wr = tr[r] > 0.f ? xt : 0.f;
+      wb[r] = wr;
+      gb[r] = -ek * wr;
+    });
+    load_row(R::RS, k);  // r_k
     float ug[NV], ur[NV], urb[NV];
     A_apply(gb, rb, ug);  // r-bar = A g-bar
-    JT_apply(rk, ur);
+    JT_apply(tr, ur);
     ltl_solve<T>(t, ur);
     jbar_add(gb, ur);
-    jbar_add(rk, ug);
+    jbar_add(tr, ug);
     mbar_sub(ug, ur);
-#pragma unroll 1
-    for (int r = 0; r < NR; ++r) Db[r] += gb[r] * rk[r];
+    for_rows([&](int r) { Db[r] += gb[r] * tr[r]; });
     A_apply(rb, ay, urb);  // y-bar = w-bar + A r-bar
     if (k > 0) {  // y_0 = 0
-      float yk[NR], uy[NV];
-      row(R::YS, k, yk);
-      JT_apply(yk, uy);
+      float uy[NV];
+      load_row(R::YS, k);  // y_k
+      JT_apply(tr, uy);
       ltl_solve<T>(t, uy);
       jbar_add(rb, uy);
-      jbar_add(yk, urb);
+      jbar_add(tr, urb);
       mbar_sub(urb, uy);
-#pragma unroll 1
-      for (int r = 0; r < NR; ++r) Db[r] += rb[r] * yk[r];
+      for_rows([&](int r) { Db[r] += rb[r] * tr[r]; });
     }
-#pragma unroll 1
-    for (int r = 0; r < NR; ++r) {
+    for_rows([&](int r) {
       yb[r] = wb[r] + ay[r];
       bb[r] += rb[r];
-    }
+    });
   }
   // ---- b = J u - aref
-  {
+  if (has_rows) {
     float jtb[NV];
     JT_apply(bb, jtb);
 #pragma unroll
@@ -292,8 +414,20 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
   for (int d = 0; d < NV; ++d) rec[R::QFB + d] = qfb[d];
 #pragma unroll
   for (int k = 0; k < R::NMB; ++k) rec[R::MB + k] = Mb[k];
+  for_rows([&](int r) {
+    const float d_ = Db[r], a_ = -bb[r];
+    rec[R::DB + r] = d_, rec[R::AB + r] = a_;
+  });
 #pragma unroll 1
-  for (int r = 0; r < NR; ++r) rec[R::DB + r] = Db[r], rec[R::AB + r] = -bb[r];
+  for (int s = 0; s < n_c; ++s) {
+    float g[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) g[k] = sb[12 * s + k];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) rec[R::SLB + 12 * s + k] = g[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 3 * NV; ++k) rec[R::CAB + k] = cab[k], rec[R::CVB + k] = cvb[k];
 #pragma unroll
   for (int k = 0; k < T::NQ; ++k) rec[R::GQ1 + k] = gq1[k];
 #pragma unroll
@@ -374,18 +508,22 @@ BGX_D float adjoint_b(const float* __restrict__ M, const float* q_in, const floa
       phi += rec[R::AB + r0 + 3] * -(bq * (e0.d - e2.d) + kt.d);
       phi += ((rec[R::DB + r0] + rec[R::DB + r0 + 1]) +
               (rec[R::DB + r0 + 2] + rec[R::DB + r0 + 3])) * diag.d;
-      const unsigned mask = link_dof_mask<T>(T::CAND_LINK[c]);
-#pragma unroll
-      for (int d = 0; d < NV; ++d) {
-        if ((mask >> d) & 1u) {
-          const V3T<S> pv = cv[d] + cross(ca[d], r);
-          const float f0 = dot(n, pv).d, f1 = mu * dot(t1, pv).d, f2 = mu * dot(t2, pv).d;
-          const int jb = R::JB + r0 * NV + d;
-          phi += rec[jb] * (f0 + f1) + rec[jb + NV] * (f0 - f1) +
-                 rec[jb + 2 * NV] * (f0 + f2) + rec[jb + 3 * NV] * (f0 - f2);
-        }
-      }
+      // J's contact rows through their ingredients (d0 = n, d1 = mu t1,
+      // d2 = mu t2, r); the cdof part is contracted once below
+      const int g = R::SLB + 12 * s;
+      const V3T<S> d1 = t1 * mu, d2 = t2 * mu;
+      if constexpr (!PZ) phi += rec[g] * n.x.d + rec[g + 1] * n.y.d + rec[g + 2] * n.z.d;
+      phi += rec[g + 3] * d1.x.d + rec[g + 4] * d1.y.d + rec[g + 5] * d1.z.d;
+      phi += rec[g + 6] * d2.x.d + rec[g + 7] * d2.y.d + rec[g + 8] * d2.z.d;
+      phi += rec[g + 9] * r.x.d + rec[g + 10] * r.y.d + rec[g + 11] * r.z.d;
     }
+  }
+#pragma unroll
+  for (int d = 0; d < NV; ++d) {
+    phi += rec[R::CAB + 3 * d] * ca[d].x.d + rec[R::CAB + 3 * d + 1] * ca[d].y.d +
+           rec[R::CAB + 3 * d + 2] * ca[d].z.d;
+    phi += rec[R::CVB + 3 * d] * cv[d].x.d + rec[R::CVB + 3 * d + 1] * cv[d].y.d +
+           rec[R::CVB + 3 * d + 2] * cv[d].z.d;
   }
 
   // limit rows

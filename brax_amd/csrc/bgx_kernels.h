@@ -180,10 +180,12 @@ __global__ __launch_bounds__(WAVE) void k_adj_fwd(AdjLaunch a) {
                     RecPtr{a.rec + env, a.n});
 }
 
-// adjoint stage A2: the reverse sweep from the tape
+// adjoint stage A2: the reverse sweep from the tape; its row vectors live in
+// LDS, rev_lanes<T>() lanes per workgroup
 template <class T>
 __global__ __launch_bounds__(WAVE) void k_adj_rev(AdjLaunch a) {
-  constexpr int NQ = T::NQ, NV = T::NV;
+  constexpr int NQ = T::NQ, NV = T::NV, AL = rev_lanes<T>();
+  __shared__ float smem[RevLds<T>::SIZE * AL];
   const int env = blockIdx.x * blockDim.x + threadIdx.x;
   if (env >= a.n) return;
   float q[NQ], qd[NV], gq1[NQ], gqd1[NV];
@@ -204,18 +206,21 @@ __global__ __launch_bounds__(WAVE) void k_adj_rev(AdjLaunch a) {
     act[k] = a.act ? a.act[(long long)env * T::NU + k] : 0.f;
     gact[k] = 0.f;
   }
-  adjoint_rev<T>(a.M, q, qd, act, gq1, gqd1, RecPtr{a.rec + env, a.n}, gact);
+  adjoint_rev<T, AL>(a.M, q, qd, act, gq1, gqd1, RecPtr{a.rec + env, a.n}, smem + threadIdx.x,
+                     gact);
 #pragma unroll
   for (int k = 0; k < T::NU; ++k) a.gact[(long long)env * T::NU + k] += gact[k];
 }
 
-// adjoint stage B: one thread per (env, input coordinate)
+// adjoint stage B: one thread per (env, input coordinate), coordinate-major
+// over env blocks of WAVE (a wave shares its coordinate; tape reads coalesce)
 template <class T>
 __global__ __launch_bounds__(WAVE) void k_adj_b(AdjLaunch a) {
   constexpr int NQ = T::NQ, NV = T::NV, ND = NQ + NV;
+  const int n_pad = (a.n + WAVE - 1) / WAVE * WAVE;
   const long long g = (long long)blockIdx.x * WAVE + threadIdx.x;
-  if (g >= (long long)a.n * ND) return;
-  const int env = (int)(g / ND), dir = (int)(g % ND);
+  const int dir = (int)(g / n_pad), env = (int)(g % n_pad);
+  if (dir >= ND || env >= a.n) return;
   float act[T::NU > 0 ? T::NU : 1];
 #pragma unroll
   for (int k = 0; k < T::NU; ++k) act[k] = a.act ? a.act[(long long)env * T::NU + k] : 0.f;
@@ -257,12 +262,12 @@ struct Ops {
     return hipGetLastError();
   }
   static hipError_t adj_rev(const AdjLaunch& a, hipStream_t s) {
-    const int b = block_lanes(a.epw);
+    const int b = block_lanes(a.epw) < rev_lanes<T>() ? block_lanes(a.epw) : rev_lanes<T>();
     hipLaunchKernelGGL(k_adj_rev<T>, env_grid(a.n, b), dim3(b), 0, s, a);
     return hipGetLastError();
   }
   static hipError_t adj_b(const AdjLaunch& a, hipStream_t s) {
-    const long long threads = (long long)a.n * (T::NQ + T::NV);
+    const long long threads = (long long)(a.n + WAVE - 1) / WAVE * WAVE * (T::NQ + T::NV);
     hipLaunchKernelGGL(k_adj_b<T>, dim3((unsigned)((threads + WAVE - 1) / WAVE)), dim3(WAVE),
                        0, s, a);
     return hipGetLastError();
