@@ -45,6 +45,7 @@ SIGNATURES = {
     "bgx_step_probe": (c_int, [c_vp, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp,
                                c_vp]),
     "bgx_adjoint_workspace_bytes": (c_i64, [c_vp, c_i32, c_i32, c_i32]),
+    "bgx_adjoint_workspace_min_bytes": (c_i64, [c_vp, c_i32, c_i32, c_i32]),
     "bgx_rollout_adjoint": (c_int, [c_vp, c_i32, c_i32, c_i32, c_i32, c_vp,
                                     c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp,
                                     c_vp, c_vp, c_i64, c_vp, c_vp]),

@@ -211,14 +211,21 @@ struct AdjWorkspace {
   float* q;       // (n, NQ) segment start, advanced by the recompute
   float* qd;      // (n, NV)
   float* adj;     // (n, NQ + NV)
-  float* rec;     // (rec, n)
+  float* rec;     // (batch, rec, n): tapes of a batch of substeps
+  int64_t batch;  // substeps taped by one stage-A1 launch
   int64_t bytes;
 };
 
+// Stage A1 tapes up to this many bytes of substeps in one launch: the
+// substeps of a checkpoint segment are independent once their start states
+// are recomputed, so one launch fills the chip even for a small batch.
+constexpr int64_t kTapeBatchBytes = 6ll << 30;
+
 int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
 
+// batch 0: as many substeps as fit in kTapeBatchBytes (at most a segment)
 AdjWorkspace adj_layout(const bgx::TopoOps* ops, float* base, int64_t n, int64_t k,
-                        int64_t n_frames) {
+                        int64_t n_frames, int64_t batch) {
   const int64_t nq = ops->dims[1], nv = ops->dims[2];
   AdjWorkspace w{};
   char* p = reinterpret_cast<char*>(base);
@@ -233,7 +240,14 @@ AdjWorkspace adj_layout(const bgx::TopoOps* ops, float* base, int64_t n, int64_t
   w.q = take(n * nq);
   w.qd = take(n * nv);
   w.adj = take(n * (nq + nv));
-  w.rec = take((int64_t)ops->adj_rec * n);
+  if (batch <= 0) {
+    const int64_t per_sub = (int64_t)ops->adj_rec * n * (int64_t)sizeof(float);
+    batch = per_sub > 0 ? kTapeBatchBytes / per_sub : 1;
+  }
+  if (batch > k * n_frames) batch = k * n_frames;
+  if (batch > 65535) batch = 65535;
+  w.batch = batch < 1 ? 1 : batch;
+  w.rec = take((int64_t)ops->adj_rec * n * w.batch);
   w.bytes = off;
   return w;
 }
@@ -243,7 +257,13 @@ AdjWorkspace adj_layout(const bgx::TopoOps* ops, float* base, int64_t n, int64_t
 int64_t bgx_adjoint_workspace_bytes(const bgx_model* m, int32_t n, int32_t ckpt_every,
                                     int32_t n_frames) {
   if (!m || !valid_topo(m->topo) || n < 0 || ckpt_every < 1 || n_frames < 1) return -1;
-  return adj_layout(kOps[m->topo], nullptr, n, ckpt_every, n_frames).bytes;
+  return adj_layout(kOps[m->topo], nullptr, n, ckpt_every, n_frames, 0).bytes;
+}
+
+int64_t bgx_adjoint_workspace_min_bytes(const bgx_model* m, int32_t n, int32_t ckpt_every,
+                                        int32_t n_frames) {
+  if (!m || !valid_topo(m->topo) || n < 0 || ckpt_every < 1 || n_frames < 1) return -1;
+  return adj_layout(kOps[m->topo], nullptr, n, ckpt_every, n_frames, 1).bytes;
 }
 
 int bgx_rollout_adjoint(const bgx_model* m, int32_t n, int32_t n_steps, int32_t n_frames,
@@ -269,8 +289,16 @@ int bgx_rollout_adjoint(const bgx_model* m, int32_t n, int32_t n_steps, int32_t 
   const bgx::TopoOps* ops = kOps[m->topo];
   const int nq = ops->dims[1], nv = ops->dims[2], nu = ops->dims[3];
   if (nu > 0 && !gact) return fail(BGX_ERR_ARG, "gact is null");
-  const AdjWorkspace w = adj_layout(ops, workspace, n, ckpt_every, n_frames);
-  if (workspace_bytes < w.bytes) return fail(BGX_ERR_ARG, "workspace too small");
+  // the most substeps per stage-A1 launch that the workspace holds (the
+  // tapes are the layout's last piece)
+  const AdjWorkspace w1 = adj_layout(ops, nullptr, n, ckpt_every, n_frames, 1);
+  if (workspace_bytes < w1.bytes) return fail(BGX_ERR_ARG, "workspace too small");
+  const int64_t per_sub = (int64_t)ops->adj_rec * n * (int64_t)sizeof(float);
+  const int64_t fixed = w1.bytes - align256(per_sub);
+  int64_t batch = per_sub > 0 ? (workspace_bytes - fixed) / per_sub : 1;
+  AdjWorkspace w = adj_layout(ops, workspace, n, ckpt_every, n_frames, batch);
+  while (w.bytes > workspace_bytes && batch > 1)
+    w = adj_layout(ops, workspace, n, ckpt_every, n_frames, --batch);
   if (!kAdjointEnabled) return fail(BGX_ERR_DISABLED, "bgx_rollout_adjoint is switched off");
   hipStream_t s = static_cast<hipStream_t>(stream);
   int rc;
@@ -298,22 +326,34 @@ int bgx_rollout_adjoint(const bgx_model* m, int32_t n, int32_t n_steps, int32_t 
     f.status = status, f.status_or = 1;
     f.epw = m->epw[BGX_KERNELS_ADJOINT];
     if ((rc = check_hip(ops->rollout(f, s), "recompute launch"))) return rc;
-    // substeps of the segment, last first
-    for (int i = ns * n_frames - 1; i >= 0; --i) {
-      const int t = s0 + i / n_frames;
-      const bool step_end = (i % n_frames) == n_frames - 1;
-      bgx::AdjLaunch a{};
-      a.M = m->table, a.n = n, a.epw = m->epw[BGX_KERNELS_ADJOINT];
-      a.q = w.sub_q + (long long)i * n * nq;
-      a.qd = w.sub_qd + (long long)i * n * nv;
-      a.act = act ? act + (long long)t * act_step_stride : nullptr;
-      a.gq_add = (step_end && gq_traj) ? gq_traj + (long long)t * n * nq : nullptr;
-      a.gqd_add = (step_end && gqd_traj) ? gqd_traj + (long long)t * n * nv : nullptr;
-      a.adj = w.adj, a.rec = w.rec;
-      a.gact = gact ? gact + (long long)t * n * nu : nullptr;
-      if ((rc = check_hip(ops->adj_fwd(a, s), "adjoint stage A1 launch"))) return rc;
-      if ((rc = check_hip(ops->adj_rev(a, s), "adjoint stage A2 launch"))) return rc;
-      if ((rc = check_hip(ops->adj_b(a, s), "adjoint stage B launch"))) return rc;
+    // substeps of the segment, last first, taped in batches of w.batch
+    const int n_sub = ns * n_frames;
+    const long long rec_stride = (long long)ops->adj_rec * n;
+    for (int hi = n_sub - 1; hi >= 0; hi -= (int)w.batch) {
+      const int lo = hi + 1 - (int)w.batch > 0 ? hi + 1 - (int)w.batch : 0;
+      bgx::AdjLaunch t{};
+      t.M = m->table, t.n = n, t.epw = m->epw[BGX_KERNELS_ADJOINT];
+      t.q = w.sub_q, t.qd = w.sub_qd;
+      t.act = act ? act + (long long)s0 * act_step_stride : nullptr;
+      t.act_stride = act_step_stride, t.n_frames = n_frames;
+      t.sub_lo = lo, t.n_sub = hi - lo + 1;
+      t.rec = w.rec, t.rec_stride = rec_stride;
+      if ((rc = check_hip(ops->adj_fwd(t, s), "adjoint stage A1 launch"))) return rc;
+      for (int i = hi; i >= lo; --i) {
+        const int t_step = s0 + i / n_frames;
+        const bool step_end = (i % n_frames) == n_frames - 1;
+        bgx::AdjLaunch a{};
+        a.M = m->table, a.n = n, a.epw = m->epw[BGX_KERNELS_ADJOINT];
+        a.q = w.sub_q + (long long)i * n * nq;
+        a.qd = w.sub_qd + (long long)i * n * nv;
+        a.act = act ? act + (long long)t_step * act_step_stride : nullptr;
+        a.gq_add = (step_end && gq_traj) ? gq_traj + (long long)t_step * n * nq : nullptr;
+        a.gqd_add = (step_end && gqd_traj) ? gqd_traj + (long long)t_step * n * nv : nullptr;
+        a.adj = w.adj, a.rec = w.rec + (long long)(i - lo) * rec_stride;
+        a.gact = gact ? gact + (long long)t_step * n * nu : nullptr;
+        if ((rc = check_hip(ops->adj_rev(a, s), "adjoint stage A2 launch"))) return rc;
+        if ((rc = check_hip(ops->adj_b(a, s), "adjoint stage B launch"))) return rc;
+      }
     }
   }
   hipLaunchKernelGGL(k_cols, grid1((long long)n * nq), dim3(256), 0, s, gq0, w.adj,

@@ -54,6 +54,11 @@ struct AdjLaunch {
   float* rec;        // (SIZE, n) stage A record
   float* gact;       // (n, NU) accumulated
   int epw;           // envs (lanes) per workgroup of A1/A2, 1..WAVE; 0 = WAVE
+  // stage A1 tapes n_sub substeps sub_lo.. of a segment in one launch: the
+  // start state of substep i at q + i n NQ (qd alike), its action at
+  // act + (i / n_frames) act_stride, its tape at rec + (i - sub_lo) rec_stride
+  int sub_lo, n_sub, n_frames;
+  long long act_stride, rec_stride;
 };
 
 // Workgroups of fewer than WAVE lanes spread a small batch over more CUs.
@@ -161,23 +166,28 @@ __global__ __launch_bounds__(WAVE) void k_rollout(Launch a) {
   if (a.status) a.status[env] = a.status_or ? (a.status[env] | st) : st;
 }
 
-// adjoint stage A1: the substep with a slot per candidate, recording its tape
+// adjoint stage A1: the substep with a slot per candidate, recording its
+// tape; blockIdx.y picks the substep of the batch
 template <class T>
 __global__ __launch_bounds__(WAVE) void k_adj_fwd(AdjLaunch a) {
   __shared__ float smem[lds_floats<T>() * WAVE];
   constexpr int NQ = T::NQ, NV = T::NV, KS = TapeLayout<T>::KS;
   const int env = blockIdx.x * blockDim.x + threadIdx.x;
   if (env >= a.n) return;
+  const int i = a.sub_lo + (int)blockIdx.y;
+  const float* qs = a.q + (long long)i * a.n * NQ;
+  const float* qds = a.qd + (long long)i * a.n * NV;
+  const float* as = a.act ? a.act + (long long)(i / a.n_frames) * a.act_stride : nullptr;
   float q[NQ], qd[NV], act[T::NU > 0 ? T::NU : 1];
 #pragma unroll
-  for (int k = 0; k < NQ; ++k) q[k] = a.q[(long long)env * NQ + k];
+  for (int k = 0; k < NQ; ++k) q[k] = qs[(long long)env * NQ + k];
 #pragma unroll
-  for (int k = 0; k < NV; ++k) qd[k] = a.qd[(long long)env * NV + k];
+  for (int k = 0; k < NV; ++k) qd[k] = qds[(long long)env * NV + k];
 #pragma unroll
-  for (int k = 0; k < T::NU; ++k) act[k] = a.act ? a.act[(long long)env * T::NU + k] : 0.f;
+  for (int k = 0; k < T::NU; ++k) act[k] = as ? as[(long long)env * T::NU + k] : 0.f;
   PrivRows<KS> rows;
   substep_ks<T, KS>(a.M, q, qd, act, smem + threadIdx.x, rows, nullptr,
-                    RecPtr{a.rec + env, a.n});
+                    RecPtr{a.rec + (long long)blockIdx.y * a.rec_stride + env, a.n});
 }
 
 // adjoint stage A2: the reverse sweep from the tape; its row vectors live in
@@ -258,7 +268,8 @@ struct Ops {
   }
   static hipError_t adj_fwd(const AdjLaunch& a, hipStream_t s) {
     const int b = block_lanes(a.epw);
-    hipLaunchKernelGGL(k_adj_fwd<T>, env_grid(a.n, b), dim3(b), 0, s, a);
+    const dim3 g = env_grid(a.n, b);
+    hipLaunchKernelGGL(k_adj_fwd<T>, dim3(g.x, (unsigned)a.n_sub), dim3(b), 0, s, a);
     return hipGetLastError();
   }
   static hipError_t adj_rev(const AdjLaunch& a, hipStream_t s) {

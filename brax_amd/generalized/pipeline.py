@@ -252,7 +252,7 @@ class _RolloutFn(torch.autograd.Function):
   recomputed from their checkpoints)."""
 
   @staticmethod
-  def forward(ctx, q0, qd0, acts, m, n_frames, ckpt_every):
+  def forward(ctx, q0, qd0, acts, m, n_frames, ckpt_every, min_workspace):
     n, T = q0.shape[0], acts.shape[0]
     q = q0.detach().contiguous().clone()
     qd = qd0.detach().contiguous().clone()
@@ -270,6 +270,7 @@ class _RolloutFn(torch.autograd.Function):
     ckqd = torch.cat([qd0.detach()[None].contiguous(), qdt[idx]], 0).contiguous()
     ctx.save_for_backward(ckq, ckqd, a)
     ctx.m, ctx.n_frames, ctx.ckpt_every = m, int(n_frames), int(ckpt_every)
+    ctx.min_workspace = bool(min_workspace)
     ctx.mark_non_differentiable(status)
     return qt, qdt, status
 
@@ -281,7 +282,9 @@ class _RolloutFn(torch.autograd.Function):
     gqt = None if gqt is None else gqt.contiguous().float()
     gqdt = None if gqdt is None else gqdt.contiguous().float()
     lib = _lib.load()
-    nbytes = lib.bgx_adjoint_workspace_bytes(m.handle, n, ctx.ckpt_every, ctx.n_frames)
+    size = (lib.bgx_adjoint_workspace_min_bytes if ctx.min_workspace
+            else lib.bgx_adjoint_workspace_bytes)
+    nbytes = size(m.handle, n, ctx.ckpt_every, ctx.n_frames)
     if nbytes < 0:
       raise _lib.BgxError("bad adjoint workspace arguments")
     ws = torch.empty(nbytes // 4 + 1, device=m.device)
@@ -294,11 +297,11 @@ class _RolloutFn(torch.autograd.Function):
         _lib.ptr(a) if m.nu else None, n * m.nu, _lib.ptr(gqt), _lib.ptr(gqdt),
         _lib.ptr(gact) if m.nu else None, _lib.ptr(gq0), _lib.ptr(gqd0), _lib.ptr(ws),
         nbytes, _lib.ptr(status), _stream()))
-    return gq0, gqd0, gact, None, None, None
+    return gq0, gqd0, gact, None, None, None, None
 
 
 def rollout_grad(sys: S.System, q0: torch.Tensor, qd0: torch.Tensor, acts: torch.Tensor,
-                 n_frames: int, ckpt_every: int = 16):
+                 n_frames: int, ckpt_every: int = 16, min_workspace: bool = False):
   """Differentiable rollout of ``acts.shape[0]`` env steps.
 
   Returns (q_traj (T, N, q_size), qd_traj (T, N, qd_size), status (N)); the
@@ -306,6 +309,8 @@ def rollout_grad(sys: S.System, q0: torch.Tensor, qd0: torch.Tensor, acts: torch
   adjoint (the reference obtains them with jax.grad through its scan of
   pipeline.step, brax/training/agents/apg/train.py:188-198).  The backward
   pass recomputes segments of ``ckpt_every`` env steps from checkpoints.
+  With ``min_workspace`` the backward pass takes the least workspace (one
+  substep's tape at a time) instead of a whole segment's tapes.
   """
   m = device_model(sys, q0.device)
   if q0.dim() != 2 or q0.shape[1] != m.nq or qd0.shape != (q0.shape[0], m.nv):
@@ -314,4 +319,5 @@ def rollout_grad(sys: S.System, q0: torch.Tensor, qd0: torch.Tensor, acts: torch
     raise ValueError(f"acts must have shape (T, {q0.shape[0]}, {m.nu})")
   if ckpt_every < 1 or n_frames < 1:
     raise ValueError("ckpt_every and n_frames must be positive")
-  return _RolloutFn.apply(q0.float(), qd0.float(), acts.float(), m, n_frames, ckpt_every)
+  return _RolloutFn.apply(q0.float(), qd0.float(), acts.float(), m, n_frames, ckpt_every,
+                          min_workspace)

@@ -121,11 +121,16 @@ int bgx_step_probe(const bgx_model* model, int32_t n_envs, int32_t n_frames,
                    float* q, float* qd, const float* act, float* probe,
                    void* stream);
 
-/* Workspace bytes bgx_rollout_adjoint needs for n_envs, a checkpoint every
- * ckpt_every env steps and n_frames substeps per env step (-1 on bad
- * arguments). */
+/* Workspace bytes for bgx_rollout_adjoint with n_envs, a checkpoint every
+ * ckpt_every env steps and n_frames substeps per env step: the preferred
+ * size (the tapes of a whole checkpoint segment, up to 6 GiB of them, so
+ * one launch tapes all its substeps), and the least size (one substep's
+ * tape per launch).  Any size from the least up works; -1 on bad
+ * arguments. */
 int64_t bgx_adjoint_workspace_bytes(const bgx_model* model, int32_t n_envs,
                                     int32_t ckpt_every, int32_t n_frames);
+int64_t bgx_adjoint_workspace_min_bytes(const bgx_model* model, int32_t n_envs,
+                                        int32_t ckpt_every, int32_t n_frames);
 
 /* Reverse-mode gradient of a rollout (bgx_rollout with n_steps env steps of
  * n_frames substeps), replacing jax.grad through the reference's scan of
@@ -142,7 +147,8 @@ int64_t bgx_adjoint_workspace_bytes(const bgx_model* model, int32_t n_envs,
  *   Outputs: gact (n_steps, N, NU), gq0 (N, NQ), gqd0 (N, NV): cotangents of
  *     the actions and of the initial state.  status (N, may be NULL)
  *     receives BGX_STATUS_* bits of the recomputed substeps.
- * workspace: device memory of bgx_adjoint_workspace_bytes() bytes.  The
+ * workspace: device memory of workspace_bytes bytes, at least
+ * bgx_adjoint_workspace_min_bytes().  The
  * call enqueues kernels only (no allocation, no synchronization).
  * Topologies with per-env model tables return BGX_ERR_TOPOLOGY. */
 int bgx_rollout_adjoint(const bgx_model* model, int32_t n_envs, int32_t n_steps,
