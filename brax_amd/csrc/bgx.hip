@@ -206,26 +206,31 @@ namespace {
 constexpr bool kAdjointEnabled = true;
 
 struct AdjWorkspace {
-  float* sub_q;   // (k * n_frames, n, NQ)
-  float* sub_qd;  // (k * n_frames, n, NV)
-  float* q;       // (n, NQ) segment start, advanced by the recompute
-  float* qd;      // (n, NV)
+  float* sub_q;   // (segs, k * n_frames, n, NQ)
+  float* sub_qd;  // (segs, k * n_frames, n, NV)
+  float* q;       // (segs, n, NQ) segment starts, advanced by the recompute
+  float* qd;      // (segs, n, NV)
   float* adj;     // (n, NQ + NV)
   float* rec;     // (batch, rec, n): tapes of a batch of substeps
+  int64_t segs;   // checkpoint segments recomputed by one launch
   int64_t batch;  // substeps taped by one stage-A1 launch
   int64_t bytes;
 };
 
+// The recompute runs up to this many bytes of segments' substep states in
+// one launch: segments are independent given their checkpoints, so one
+// launch recomputes them side by side.
+constexpr int64_t kSegBatchBytes = 1ll << 30;
 // Stage A1 tapes up to this many bytes of substeps in one launch: the
-// substeps of a checkpoint segment are independent once their start states
-// are recomputed, so one launch fills the chip even for a small batch.
+// substeps of a segment are independent once their start states are
+// recomputed, so one launch fills the chip even for a small batch.
 constexpr int64_t kTapeBatchBytes = 6ll << 30;
 
 int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
 
-// batch 0: as many substeps as fit in kTapeBatchBytes (at most a segment)
+// segs / batch 0: as many as fit in kSegBatchBytes / kTapeBatchBytes
 AdjWorkspace adj_layout(const bgx::TopoOps* ops, float* base, int64_t n, int64_t k,
-                        int64_t n_frames, int64_t batch) {
+                        int64_t n_frames, int64_t segs, int64_t batch) {
   const int64_t nq = ops->dims[1], nv = ops->dims[2];
   AdjWorkspace w{};
   char* p = reinterpret_cast<char*>(base);
@@ -235,11 +240,12 @@ AdjWorkspace adj_layout(const bgx::TopoOps* ops, float* base, int64_t n, int64_t
     off += align256(floats * (int64_t)sizeof(float));
     return r;
   };
-  w.sub_q = take(k * n_frames * n * nq);
-  w.sub_qd = take(k * n_frames * n * nv);
-  w.q = take(n * nq);
-  w.qd = take(n * nv);
-  w.adj = take(n * (nq + nv));
+  if (segs <= 0) {
+    const int64_t per_seg = (k * n_frames + 1) * n * (nq + nv) * (int64_t)sizeof(float);
+    segs = per_seg > 0 ? kSegBatchBytes / per_seg : 1;
+  }
+  if (segs > 65535) segs = 65535;
+  w.segs = segs < 1 ? 1 : segs;
   if (batch <= 0) {
     const int64_t per_sub = (int64_t)ops->adj_rec * n * (int64_t)sizeof(float);
     batch = per_sub > 0 ? kTapeBatchBytes / per_sub : 1;
@@ -247,6 +253,11 @@ AdjWorkspace adj_layout(const bgx::TopoOps* ops, float* base, int64_t n, int64_t
   if (batch > k * n_frames) batch = k * n_frames;
   if (batch > 65535) batch = 65535;
   w.batch = batch < 1 ? 1 : batch;
+  w.sub_q = take(w.segs * k * n_frames * n * nq);
+  w.sub_qd = take(w.segs * k * n_frames * n * nv);
+  w.q = take(w.segs * n * nq);
+  w.qd = take(w.segs * n * nv);
+  w.adj = take(n * (nq + nv));
   w.rec = take((int64_t)ops->adj_rec * n * w.batch);
   w.bytes = off;
   return w;
@@ -257,13 +268,13 @@ AdjWorkspace adj_layout(const bgx::TopoOps* ops, float* base, int64_t n, int64_t
 int64_t bgx_adjoint_workspace_bytes(const bgx_model* m, int32_t n, int32_t ckpt_every,
                                     int32_t n_frames) {
   if (!m || !valid_topo(m->topo) || n < 0 || ckpt_every < 1 || n_frames < 1) return -1;
-  return adj_layout(kOps[m->topo], nullptr, n, ckpt_every, n_frames, 0).bytes;
+  return adj_layout(kOps[m->topo], nullptr, n, ckpt_every, n_frames, 0, 0).bytes;
 }
 
 int64_t bgx_adjoint_workspace_min_bytes(const bgx_model* m, int32_t n, int32_t ckpt_every,
                                         int32_t n_frames) {
   if (!m || !valid_topo(m->topo) || n < 0 || ckpt_every < 1 || n_frames < 1) return -1;
-  return adj_layout(kOps[m->topo], nullptr, n, ckpt_every, n_frames, 1).bytes;
+  return adj_layout(kOps[m->topo], nullptr, n, ckpt_every, n_frames, 1, 1).bytes;
 }
 
 int bgx_rollout_adjoint(const bgx_model* m, int32_t n, int32_t n_steps, int32_t n_frames,
@@ -289,16 +300,20 @@ int bgx_rollout_adjoint(const bgx_model* m, int32_t n, int32_t n_steps, int32_t 
   const bgx::TopoOps* ops = kOps[m->topo];
   const int nq = ops->dims[1], nv = ops->dims[2], nu = ops->dims[3];
   if (nu > 0 && !gact) return fail(BGX_ERR_ARG, "gact is null");
-  // the most substeps per stage-A1 launch that the workspace holds (the
-  // tapes are the layout's last piece)
-  const AdjWorkspace w1 = adj_layout(ops, nullptr, n, ckpt_every, n_frames, 1);
+  // segments per recompute launch: the preferred count if the workspace
+  // holds it with one tape, else one; then the most substeps per stage-A1
+  // launch that the rest holds (the tapes are the layout's last piece)
+  const AdjWorkspace w1 = adj_layout(ops, nullptr, n, ckpt_every, n_frames, 1, 1);
   if (workspace_bytes < w1.bytes) return fail(BGX_ERR_ARG, "workspace too small");
+  const AdjWorkspace wp = adj_layout(ops, nullptr, n, ckpt_every, n_frames, 0, 1);
+  const int64_t segs = workspace_bytes >= wp.bytes ? wp.segs : 1;
+  const AdjWorkspace ws = adj_layout(ops, nullptr, n, ckpt_every, n_frames, segs, 1);
   const int64_t per_sub = (int64_t)ops->adj_rec * n * (int64_t)sizeof(float);
-  const int64_t fixed = w1.bytes - align256(per_sub);
+  const int64_t fixed = ws.bytes - align256(per_sub);
   int64_t batch = per_sub > 0 ? (workspace_bytes - fixed) / per_sub : 1;
-  AdjWorkspace w = adj_layout(ops, workspace, n, ckpt_every, n_frames, batch);
+  AdjWorkspace w = adj_layout(ops, workspace, n, ckpt_every, n_frames, segs, batch);
   while (w.bytes > workspace_bytes && batch > 1)
-    w = adj_layout(ops, workspace, n, ckpt_every, n_frames, --batch);
+    w = adj_layout(ops, workspace, n, ckpt_every, n_frames, segs, --batch);
   if (!kAdjointEnabled) return fail(BGX_ERR_DISABLED, "bgx_rollout_adjoint is switched off");
   hipStream_t s = static_cast<hipStream_t>(stream);
   int rc;
@@ -309,52 +324,65 @@ int bgx_rollout_adjoint(const bgx_model* m, int32_t n, int32_t n_steps, int32_t 
   if ((rc = check_hip(hipGetLastError(), "fill"))) return rc;
 
   const int n_seg = (n_steps + ckpt_every - 1) / ckpt_every;
-  for (int seg = n_seg - 1; seg >= 0; --seg) {
-    const int s0 = seg * ckpt_every;
-    const int ns = n_steps - s0 < ckpt_every ? n_steps - s0 : ckpt_every;
-    // recompute the states at the start of every substep of the segment
-    hipLaunchKernelGGL(k_copy, grid1((long long)n * nq), dim3(256), 0, s, w.q,
-                       ckpt_q + (long long)seg * n * nq, (long long)n * nq);
-    hipLaunchKernelGGL(k_copy, grid1((long long)n * nv), dim3(256), 0, s, w.qd,
-                       ckpt_qd + (long long)seg * n * nv, (long long)n * nv);
+  for (int seg_hi = n_seg - 1; seg_hi >= 0;) {
+    // a group of segments, recomputed side by side by one launch (a short
+    // last segment goes alone)
+    const bool short_last = seg_hi == n_seg - 1 && n_steps % ckpt_every != 0;
+    const int cnt = short_last ? 1 : (seg_hi + 1 < (int)w.segs ? seg_hi + 1 : (int)w.segs);
+    const int seg_lo = seg_hi - cnt + 1;
+    const int ns_group = short_last ? n_steps - seg_hi * ckpt_every : ckpt_every;
+    hipLaunchKernelGGL(k_copy, grid1((long long)cnt * n * nq), dim3(256), 0, s, w.q,
+                       ckpt_q + (long long)seg_lo * n * nq, (long long)cnt * n * nq);
+    hipLaunchKernelGGL(k_copy, grid1((long long)cnt * n * nv), dim3(256), 0, s, w.qd,
+                       ckpt_qd + (long long)seg_lo * n * nv, (long long)cnt * n * nv);
     bgx::Launch f{};
-    f.M = m->table, f.n = n, f.n_steps = ns, f.n_frames = n_frames;
+    f.M = m->table, f.n = n, f.n_steps = ns_group, f.n_frames = n_frames;
     f.q = w.q, f.qd = w.qd;
-    f.act = act ? act + (long long)s0 * act_step_stride : nullptr;
+    f.act = act ? act + (long long)seg_lo * ckpt_every * act_step_stride : nullptr;
     f.act_stride = act_step_stride;
+    f.n_seg = cnt, f.seg_act_stride = (long long)ckpt_every * act_step_stride;
     f.sub_q = w.sub_q, f.sub_qd = w.sub_qd;
     f.status = status, f.status_or = 1;
     f.epw = m->epw[BGX_KERNELS_ADJOINT];
     if ((rc = check_hip(ops->rollout(f, s), "recompute launch"))) return rc;
-    // substeps of the segment, last first, taped in batches of w.batch
-    const int n_sub = ns * n_frames;
-    const long long rec_stride = (long long)ops->adj_rec * n;
-    for (int hi = n_sub - 1; hi >= 0; hi -= (int)w.batch) {
-      const int lo = hi + 1 - (int)w.batch > 0 ? hi + 1 - (int)w.batch : 0;
-      bgx::AdjLaunch t{};
-      t.M = m->table, t.n = n, t.epw = m->epw[BGX_KERNELS_ADJOINT];
-      t.q = w.sub_q, t.qd = w.sub_qd;
-      t.act = act ? act + (long long)s0 * act_step_stride : nullptr;
-      t.act_stride = act_step_stride, t.n_frames = n_frames;
-      t.sub_lo = lo, t.n_sub = hi - lo + 1;
-      t.rec = w.rec, t.rec_stride = rec_stride;
-      if ((rc = check_hip(ops->adj_fwd(t, s), "adjoint stage A1 launch"))) return rc;
-      for (int i = hi; i >= lo; --i) {
-        const int t_step = s0 + i / n_frames;
-        const bool step_end = (i % n_frames) == n_frames - 1;
-        bgx::AdjLaunch a{};
-        a.M = m->table, a.n = n, a.epw = m->epw[BGX_KERNELS_ADJOINT];
-        a.q = w.sub_q + (long long)i * n * nq;
-        a.qd = w.sub_qd + (long long)i * n * nv;
-        a.act = act ? act + (long long)t_step * act_step_stride : nullptr;
-        a.gq_add = (step_end && gq_traj) ? gq_traj + (long long)t_step * n * nq : nullptr;
-        a.gqd_add = (step_end && gqd_traj) ? gqd_traj + (long long)t_step * n * nv : nullptr;
-        a.adj = w.adj, a.rec = w.rec + (long long)(i - lo) * rec_stride;
-        a.gact = gact ? gact + (long long)t_step * n * nu : nullptr;
-        if ((rc = check_hip(ops->adj_rev(a, s), "adjoint stage A2 launch"))) return rc;
-        if ((rc = check_hip(ops->adj_b(a, s), "adjoint stage B launch"))) return rc;
+    for (int seg = seg_hi; seg >= seg_lo; --seg) {
+      const int s0 = seg * ckpt_every;
+      // substep states of this segment (the launch's segment stride is its
+      // own length ns_group)
+      const long long sub_off = (long long)(seg - seg_lo) * ns_group * n_frames * n;
+      float* sq = w.sub_q + sub_off * nq;
+      float* sqd = w.sub_qd + sub_off * nv;
+      // substeps of the segment, last first, taped in batches of w.batch
+      const int n_sub = ns_group * n_frames;
+      const long long rec_stride = (long long)ops->adj_rec * n;
+      for (int hi = n_sub - 1; hi >= 0; hi -= (int)w.batch) {
+        const int lo = hi + 1 - (int)w.batch > 0 ? hi + 1 - (int)w.batch : 0;
+        bgx::AdjLaunch t{};
+        t.M = m->table, t.n = n, t.epw = m->epw[BGX_KERNELS_ADJOINT];
+        t.q = sq, t.qd = sqd;
+        t.act = act ? act + (long long)s0 * act_step_stride : nullptr;
+        t.act_stride = act_step_stride, t.n_frames = n_frames;
+        t.sub_lo = lo, t.n_sub = hi - lo + 1;
+        t.rec = w.rec, t.rec_stride = rec_stride;
+        if ((rc = check_hip(ops->adj_fwd(t, s), "adjoint stage A1 launch"))) return rc;
+        for (int i = hi; i >= lo; --i) {
+          const int t_step = s0 + i / n_frames;
+          const bool step_end = (i % n_frames) == n_frames - 1;
+          bgx::AdjLaunch a{};
+          a.M = m->table, a.n = n, a.epw = m->epw[BGX_KERNELS_ADJOINT];
+          a.q = sq + (long long)i * n * nq;
+          a.qd = sqd + (long long)i * n * nv;
+          a.act = act ? act + (long long)t_step * act_step_stride : nullptr;
+          a.gq_add = (step_end && gq_traj) ? gq_traj + (long long)t_step * n * nq : nullptr;
+          a.gqd_add = (step_end && gqd_traj) ? gqd_traj + (long long)t_step * n * nv : nullptr;
+          a.adj = w.adj, a.rec = w.rec + (long long)(i - lo) * rec_stride;
+          a.gact = gact ? gact + (long long)t_step * n * nu : nullptr;
+          if ((rc = check_hip(ops->adj_rev(a, s), "adjoint stage A2 launch"))) return rc;
+          if ((rc = check_hip(ops->adj_b(a, s), "adjoint stage B launch"))) return rc;
+        }
       }
     }
+    seg_hi = seg_lo - 1;
   }
   hipLaunchKernelGGL(k_cols, grid1((long long)n * nq), dim3(256), 0, s, gq0, w.adj,
                      (long long)n, nq, nq + nv, 0);

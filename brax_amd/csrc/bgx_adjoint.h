@@ -438,9 +438,10 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
 // stage B: d/de of <cotangents, ingredients> along input coordinate dir
 // (dir < NQ: q[dir]; else qd[dir - NQ]); returns the input cotangent
 
+// along q[dir]: every ingredient moves
 template <class T>
-BGX_D float adjoint_b(const float* __restrict__ M, const float* q_in, const float* qd_in,
-                      const float* act, RecPtr rec, int dir) {
+BGX_D float adjoint_b_q(const float* __restrict__ M, const float* q_in, const float* qd_in,
+                        const float* act, RecPtr rec, int dir) {
   using R = AdjRec<T>;
   using S = Dual;
   constexpr int NV = T::NV, NQ = T::NQ, L = T::L, NRC = R::NRC;
@@ -565,9 +566,99 @@ BGX_D float adjoint_b(const float* __restrict__ M, const float* q_in, const floa
           if (k < T::LTYPE[l]) phi += rec[R::GQ1 + qa + k] * q[qa + k].d;
       }
     }
-    return phi;
   }
-  return phi + rec[R::GQDT + (dir - NQ)];
+  return phi;
+}
+
+BGX_D Sym3T<Dual> lift_sym(const Sym3& a) {
+  return Sym3T<Dual>{lift<Dual>(a.xx), lift<Dual>(a.yy), lift<Dual>(a.zz),
+                     lift<Dual>(a.xy), lift<Dual>(a.xz), lift<Dual>(a.yz)};
+}
+
+// along qd[j]: the geometry (frames, cdof, M, J, the contact frames) stays
+// put; only qf_smooth, aref's velocity terms and the pass-through move
+template <class T>
+BGX_D float adjoint_b_qd(const float* __restrict__ M, const float* q_in, const float* qd_in,
+                         const float* act, RecPtr rec, int j) {
+  using R = AdjRec<T>;
+  using S = Dual;
+  constexpr int NV = T::NV, NQ = T::NQ, L = T::L, NRC = R::NRC;
+  constexpr bool PZ = T::PLANE_Z != 0;
+  V3 xp[L];
+  Q4 xr[L];
+  forward_kinematics<T, false, float>(M, q_in, qd_in, xp, xr, nullptr, nullptr);
+  const V3 rc = root_com<T>(M, xp, xr);
+  S q[NQ], qd[NV];
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) q[k] = lift<S>(q_in[k]);
+#pragma unroll
+  for (int k = 0; k < NV; ++k) qd[k] = dual(qd_in[k], k == j ? 1.f : 0.f);
+  V3T<S> ca[NV], cv[NV], cc[L], cda[L], cdv[L];
+  Sym3T<S> ci[L];
+  {
+    V3 ccf[L], caf[NV], cvf[NV];
+    Sym3 cif[L];
+    com_inertia<T>(M, xp, xr, rc, ccf, cif);
+    dof_motions<T>(M, q_in, xp, xr, rc, caf, cvf);
+#pragma unroll
+    for (int l = 0; l < L; ++l) cc[l] = lift3<S>(ccf[l]), ci[l] = lift_sym(cif[l]);
+#pragma unroll
+    for (int d = 0; d < NV; ++d) ca[d] = lift3<S>(caf[d]), cv[d] = lift3<S>(cvf[d]);
+  }
+  link_vel<T>(ca, cv, qd, cda, cdv);
+  S qf[NV];
+  rnea_bias<T>(M, ca, cv, cda, cdv, cc, ci, qd, qf);
+  smooth_forces<T>(M, q, qd, act, qf);
+  float phi = 0.f;
+#pragma unroll
+  for (int d = 0; d < NV; ++d) phi += rec[R::QFB + d] * qf[d].d;
+
+  // contact rows: aref = -b (J qd) - k imp dist, its velocity term
+#pragma unroll
+  for (int c = 0; c < T::NCAND; ++c) {
+    const int s = (int)rec[R::CSLOT + c];
+    if (s >= 0) {
+      const int l = T::CAND_LINK[c];
+      const float* cd = cand_rec<T>(M, c);
+      V3 ctr, axis, pos, t1, t2;
+      candidate_center<T>(M, c, xp, xr, ctr, axis);
+      const float dist = candidate_dist<T>(M, c, ctr);
+      contact_frame<T>(M, c, ctr, axis, dist, (int)rec[R::CFB + c], pos, t1, t2);
+      const V3 n = PZ ? v3(0.f, 0.f, 1.f) : ldv3(cd + CD_PN);
+      const float mu = cd[CD_MU];
+      const V3T<S> r = lift3<S>(pos - rc);
+      const float bq = cd[CD_IMP + IM_B];
+      const V3T<S> vq = cdv[l] + cross(cda[l], r);
+      const float e0 = dot(lift3<S>(n), vq).d, e1 = mu * dot(lift3<S>(t1), vq).d;
+      const float e2 = mu * dot(lift3<S>(t2), vq).d;
+      const int r0 = 4 * s;
+      phi -= bq * (rec[R::AB + r0] * (e0 + e1) + rec[R::AB + r0 + 1] * (e0 - e1) +
+                   rec[R::AB + r0 + 2] * (e0 + e2) + rec[R::AB + r0 + 3] * (e0 - e2));
+    }
+  }
+  // limit rows: aref = -b side qd[d] - k imp pos
+#pragma unroll
+  for (int i = 0; i < T::NLIM; ++i) {
+    const int d = T::LIM_D[i], qi = T::LIM_Q[i];
+    if (d == j) {
+      const float* df = dof_rec<T>(M, d);
+      const float pmin = q_in[qi] - df[DF_LO];
+      const float pmax = df[DF_HI] - q_in[qi];
+      const float pos = fminf(fminf(pmin, pmax), 0.f);
+      if (pos < 0.f) {
+        const float side = pmin < pmax ? 1.f : -1.f;
+        phi += rec[R::AB + NRC + i] * (-df[DF_IMP + IM_B] * side);
+      }
+    }
+  }
+  return phi + rec[R::GQDT + j];
+}
+
+template <class T>
+BGX_D float adjoint_b(const float* __restrict__ M, const float* q_in, const float* qd_in,
+                      const float* act, RecPtr rec, int dir) {
+  return dir < T::NQ ? adjoint_b_q<T>(M, q_in, qd_in, act, rec, dir)
+                     : adjoint_b_qd<T>(M, q_in, qd_in, act, rec, dir - T::NQ);
 }
 
 }  // namespace bgx

@@ -38,6 +38,12 @@ struct Launch {
   float* sub_q;   // (n_steps * n_frames, n, NQ): state at the start of each substep
   float* sub_qd;  // (n_steps * n_frames, n, NV)
   int epw;        // envs (lanes) per workgroup, 1..WAVE; 0 = WAVE
+  // n_seg > 1 (the adjoint's recompute): n_seg independent rollouts of the
+  // same length, blockIdx.y = j: state at q + j n NQ (qd alike), actions
+  // from act + j seg_act_stride, substep states at sub_q + j n_steps
+  // n_frames n NQ (sub_qd alike), status bits ORed atomically
+  int n_seg;
+  long long seg_act_stride;
 };
 
 // one adjoint substep: state (q, qd) of substep i, its action, the running
@@ -116,26 +122,33 @@ __global__ __launch_bounds__(WAVE) void k_rollout(Launch a) {
   __shared__ float smem[lds_floats<T>() * WAVE];  // layout: lds_floats()
   const int env = blockIdx.x * blockDim.x + threadIdx.x;
   if (env >= a.n) return;
+  const long long seg = blockIdx.y;
+  float* qp = a.q + seg * a.n * T::NQ;
+  float* qdp = a.qd + seg * a.n * T::NV;
+  const float* actp = a.act ? a.act + seg * a.seg_act_stride : nullptr;
+  const long long n_sub = (long long)a.n_steps * a.n_frames;
+  float* subq = a.sub_q ? a.sub_q + seg * n_sub * a.n * T::NQ : nullptr;
+  float* subqd = a.sub_qd ? a.sub_qd + seg * n_sub * a.n * T::NV : nullptr;
   float* lds = smem + threadIdx.x;
   float q[T::NQ], qd[T::NV];
 #pragma unroll
-  for (int k = 0; k < T::NQ; ++k) q[k] = a.q[(long long)env * T::NQ + k];
+  for (int k = 0; k < T::NQ; ++k) q[k] = qp[(long long)env * T::NQ + k];
 #pragma unroll
-  for (int k = 0; k < T::NV; ++k) qd[k] = a.qd[(long long)env * T::NV + k];
+  for (int k = 0; k < T::NV; ++k) qd[k] = qdp[(long long)env * T::NV + k];
   float* probe = a.probe ? a.probe + (long long)env * (2 * T::NV + 3) : nullptr;
   int st = 0;
   for (int s = 0; s < a.n_steps; ++s) {
     float act[T::NU > 0 ? T::NU : 1];
 #pragma unroll
     for (int k = 0; k < T::NU; ++k)
-      act[k] = a.act ? a.act[s * a.act_stride + (long long)env * T::NU + k] : 0.f;
+      act[k] = actp ? actp[s * a.act_stride + (long long)env * T::NU + k] : 0.f;
     for (int f = 0; f < a.n_frames; ++f) {
-      if (a.sub_q) {
+      if (subq) {
         const long long i = (long long)s * a.n_frames + f;
 #pragma unroll
-        for (int k = 0; k < T::NQ; ++k) a.sub_q[(i * a.n + env) * T::NQ + k] = q[k];
+        for (int k = 0; k < T::NQ; ++k) subq[(i * a.n + env) * T::NQ + k] = q[k];
 #pragma unroll
-        for (int k = 0; k < T::NV; ++k) a.sub_qd[(i * a.n + env) * T::NV + k] = qd[k];
+        for (int k = 0; k < T::NV; ++k) subqd[(i * a.n + env) * T::NV + k] = qd[k];
       }
       st |= substep<T>(env_model<T>(a.M, env), q, qd, act, lds, probe);
     }
@@ -154,16 +167,19 @@ __global__ __launch_bounds__(WAVE) void k_rollout(Launch a) {
 #pragma unroll
   for (int k = 0; k < T::NQ; ++k) {
     finite = finite && isfinite(q[k]);
-    a.q[(long long)env * T::NQ + k] = q[k];
+    qp[(long long)env * T::NQ + k] = q[k];
   }
 #pragma unroll
   for (int k = 0; k < T::NV; ++k) {
     finite = finite && isfinite(qd[k]);
-    a.qd[(long long)env * T::NV + k] = qd[k];
+    qdp[(long long)env * T::NV + k] = qd[k];
   }
   if (!finite) st |= ST_NONFINITE;
   if (a.x_pos || a.x_rot || a.xd_ang || a.xd_vel) write_frames<T>(a, env, q, qd);
-  if (a.status) a.status[env] = a.status_or ? (a.status[env] | st) : st;
+  if (a.status) {
+    if (a.n_seg > 1) atomicOr(a.status + env, st);
+    else a.status[env] = a.status_or ? (a.status[env] | st) : st;
+  }
 }
 
 // adjoint stage A1: the substep with a slot per candidate, recording its
@@ -263,7 +279,9 @@ struct Ops {
   }
   static hipError_t rollout(const Launch& a, hipStream_t s) {
     const int b = block_lanes(a.epw);
-    hipLaunchKernelGGL(k_rollout<T>, env_grid(a.n, b), dim3(b), 0, s, a);
+    const dim3 g = env_grid(a.n, b);
+    hipLaunchKernelGGL(k_rollout<T>, dim3(g.x, a.n_seg > 1 ? (unsigned)a.n_seg : 1u), dim3(b),
+                       0, s, a);
     return hipGetLastError();
   }
   static hipError_t adj_fwd(const AdjLaunch& a, hipStream_t s) {
