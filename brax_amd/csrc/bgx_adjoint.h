@@ -62,8 +62,8 @@ struct RevLds {
   using R = AdjRec<T>;
   static constexpr int NR = R::NR, KS = R::KS, NV = T::NV;
   static constexpr int XB = 0, YB = XB + NR, BB = YB + NR, DB = BB + NR, WB = DB + NR,
-                       GB = WB + NR, RB = GB + NR, TR = RB + NR;
-  static constexpr int SL = TR + NR;       // [KS][14]
+                       GB = WB + NR, RB = GB + NR, TR = RB + NR, TR2 = TR + NR;
+  static constexpr int SL = TR2 + NR;      // [KS][14]
   static constexpr int SB = SL + 14 * KS;  // [KS][12]
   static constexpr int CAB = SB + 12 * KS;  // [3 NV]
   static constexpr int CVB = CAB + 3 * NV;  // [3 NV]
@@ -86,6 +86,16 @@ struct LVec {
   BGX_D float& operator[](int r) const { return p[r * S]; }
 };
 
+// lanes per env of the reverse sweep: two lanes run the same instructions
+// on different products of each FISTA iteration, then add their sums
+template <class T>
+constexpr int rev_pair() {
+  return 2;
+}
+
+// the value of the other lane of an env's lane pair
+BGX_D float pair_other(float v) { return __shfl_xor(v, 1); }
+
 BGX_D V3 ld3(const RecPtr& rec, int f) { return v3(rec[f], rec[f + 1], rec[f + 2]); }
 
 // ---------------------------------------------------------------------------
@@ -94,17 +104,28 @@ BGX_D V3 ld3(const RecPtr& rec, int f) { return v3(rec[f], rec[f + 1], rec[f + 2
 // the active rows only (contact rows 4 s + k of the n_c slots, then the
 // static limit rows); the row vectors and slot data sit in LDS.
 
-template <class T, int AL>
+// P = 1: one lane per env.  P = 2: a pair of lanes per env (sub = 0, 1)
+// computes the same sweep; the products of J's and M's cotangent that a
+// FISTA iteration adds are split between the pair (sub 0: the ones that
+// pair g-bar with M^-1 J^T r_k and r-bar with M^-1 J^T y_k; sub 1: r_k with
+// M^-1 J^T g-bar and y_k with M^-1 J^T r-bar), each lane sums its own,
+// and the pair adds its sums at the end.  Every lane of the pair returns
+// the same result.
+template <class T, int AL, int P = 1>
 BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float* qd,
                        const float* act, const float* gq1, const float* gqd1, RecPtr rec,
-                       float* lds, float* gact) {
+                       float* lds, float* gact, int sub = 0) {
   using R = AdjRec<T>;
   using Y = RevLds<T>;
   constexpr int NV = T::NV, L = T::L, NR = R::NR, NRC = R::NRC;
+  static_assert(P == 1 || P == 2, "one or two lanes per env");
   const int n_c = (int)rec[R::NC], iters = (int)rec[R::ITERS];
+  // products both lanes form alike are counted by lane 0 only
+  const float own = (P == 1 || sub == 0) ? 1.f : 0.f;
   auto vec = [&](int off) { return LVec<AL>{lds + off * AL}; };
   const LVec<AL> xb = vec(Y::XB), yb = vec(Y::YB), bb = vec(Y::BB), Db = vec(Y::DB),
-                 wb = vec(Y::WB), gb = vec(Y::GB), rb = vec(Y::RB), tr = vec(Y::TR);
+                 wb = vec(Y::WB), gb = vec(Y::GB), rb = vec(Y::RB), tr = vec(Y::TR),
+                 tr2 = vec(Y::TR2);
   const LVec<AL> sl = vec(Y::SL), sb = vec(Y::SB), cab = vec(Y::CAB), cvb = vec(Y::CVB);
   const LVec<AL> ay = gb;  // A r-bar; g-bar is dead by then
 
@@ -160,20 +181,20 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
 #pragma unroll
     for (int i = 0; i < T::NLIM; ++i) f(NRC + i);
   };
-  // tape row vector of iteration k into tr (4 loads in flight per slot)
-  auto load_row = [&](int base, int k) {
+  // tape row vector of iteration k into dst (4 loads in flight per slot)
+  auto load_row = [&](const LVec<AL>& dst, int base, int k) {
     const int b0 = base + k * NR;
 #pragma unroll 1
     for (int s = 0; s < n_c; ++s) {
       const float v0 = rec[b0 + 4 * s], v1 = rec[b0 + 4 * s + 1];
       const float v2 = rec[b0 + 4 * s + 2], v3_ = rec[b0 + 4 * s + 3];
-      tr[4 * s] = v0, tr[4 * s + 1] = v1, tr[4 * s + 2] = v2, tr[4 * s + 3] = v3_;
+      dst[4 * s] = v0, dst[4 * s + 1] = v1, dst[4 * s + 2] = v2, dst[4 * s + 3] = v3_;
     }
     float lv[R::NLR];
 #pragma unroll
     for (int i = 0; i < T::NLIM; ++i) lv[i] = rec[b0 + NRC + i];
 #pragma unroll
-    for (int i = 0; i < T::NLIM; ++i) tr[NRC + i] = lv[i];
+    for (int i = 0; i < T::NLIM; ++i) dst[NRC + i] = lv[i];
   };
 
   // J z on the active rows
@@ -338,11 +359,14 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
     w[d] = rec[R::W + d];
   }
   if (has_rows && iters > 0) {
-    load_row(R::XS, iters - 1);
+    float zo[NV];
+    load_row(tr, R::XS, iters - 1);
     ltl_solve<T>(t, z);
-    mbar_sub(z, w);
+#pragma unroll
+    for (int d = 0; d < NV; ++d) zo[d] = own * z[d];
+    mbar_sub(zo, w);
     J_apply(z, xb);
-    jbar_add(tr, z);
+    jbar_add(tr, zo);
   }
   // ---- unrolled FISTA, last iteration first:
   //   r_k = A y_k + b, g_k = A r_k, x_{k+1} = relu(y_k - eta_k g_k),
@@ -350,7 +374,7 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
 #pragma unroll 1
   for (int k = iters - 1; k >= 0; --k) {
     const float bk = rec[R::BETA + k], ek = rec[R::ETA + k];
-    load_row(R::XS, k);  // x_{k+1}
+    load_row(tr, R::XS, k);  // x_{k+1}
     for_rows([&](int r) {
       const float xt = xb[r] + (1.f + bk) * yb[r];
       xb[r] = -bk * yb[r];
@@ -358,25 +382,49 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
       wb[r] = wr;
       gb[r] = -ek * wr;
     });
-    load_row(R::RS, k);  // r_k
-    float ug[NV], ur[NV], urb[NV];
-    A_apply(gb, rb, ug);  // r-bar = A g-bar
-    JT_apply(tr, ur);
-    ltl_solve<T>(t, ur);
-    jbar_add(gb, ur);
-    jbar_add(tr, ug);
-    mbar_sub(ug, ur);
-    for_rows([&](int r) { Db[r] += gb[r] * tr[r]; });
-    A_apply(rb, ay, urb);  // y-bar = w-bar + A r-bar
-    if (k > 0) {  // y_0 = 0
-      float uy[NV];
-      load_row(R::YS, k);  // y_k
-      JT_apply(tr, uy);
-      ltl_solve<T>(t, uy);
-      jbar_add(rb, uy);
-      jbar_add(tr, urb);
-      mbar_sub(urb, uy);
-      for_rows([&](int r) { Db[r] += rb[r] * tr[r]; });
+    load_row(tr, R::RS, k);  // r_k
+    if constexpr (P == 1) {
+      float ug[NV], ur[NV], urb[NV];
+      A_apply(gb, rb, ug);  // r-bar = A g-bar
+      JT_apply(tr, ur);
+      ltl_solve<T>(t, ur);
+      jbar_add(gb, ur);
+      jbar_add(tr, ug);
+      mbar_sub(ug, ur);
+      for_rows([&](int r) { Db[r] += gb[r] * tr[r]; });
+      A_apply(rb, ay, urb);  // y-bar = w-bar + A r-bar
+      if (k > 0) {  // y_0 = 0
+        float uy[NV];
+        load_row(tr, R::YS, k);  // y_k
+        JT_apply(tr, uy);
+        ltl_solve<T>(t, uy);
+        jbar_add(rb, uy);
+        jbar_add(tr, urb);
+        mbar_sub(urb, uy);
+        for_rows([&](int r) { Db[r] += rb[r] * tr[r]; });
+      }
+    } else {
+      if (k > 0) load_row(tr2, R::YS, k);  // y_k
+      else for_rows([&](int r) { tr2[r] = 0.f; });  // y_0 = 0
+      float ug[NV], urb[NV], uo[NV], ux[NV], c[NV];
+      A_apply(gb, rb, ug);  // r-bar = A g-bar
+      // sub 0: ur = M^-1 J^T r_k;  sub 1: uy = M^-1 J^T y_k
+      JT_apply(sub ? tr2 : tr, uo);
+      ltl_solve<T>(t, uo);
+#pragma unroll
+      for (int d = 0; d < NV; ++d) c[d] = sub ? ug[d] : uo[d];
+      jbar_add(sub ? tr : gb, c);  // (g-bar, ur) | (r_k, ug)
+      for_rows([&](int r) { Db[r] += gb[r] * tr[r]; });
+      A_apply(rb, ay, urb);  // y-bar = w-bar + A r-bar
+#pragma unroll
+      for (int d = 0; d < NV; ++d) ux[d] = pair_other(uo[d]);  // sub 0 gets uy
+#pragma unroll
+      for (int d = 0; d < NV; ++d) c[d] = sub ? urb[d] : ux[d];
+      jbar_add(sub ? tr2 : rb, c);  // (r-bar, uy) | (y_k, urb)
+#pragma unroll
+      for (int d = 0; d < NV; ++d) c[d] = sub ? urb[d] : ug[d];
+      mbar_sub(c, uo);  // (ug, ur) | (urb, uy)
+      for_rows([&](int r) { Db[r] += rb[r] * tr2[r]; });
     }
     for_rows([&](int r) {
       yb[r] = wb[r] + ay[r];
@@ -385,18 +433,33 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
   }
   // ---- b = J u - aref
   if (has_rows) {
-    float jtb[NV];
+    float jtb[NV], uo[NV];
     JT_apply(bb, jtb);
 #pragma unroll
-    for (int d = 0; d < NV; ++d) ub[d] += jtb[d];
-    jbar_add(bb, u);
+    for (int d = 0; d < NV; ++d) ub[d] += jtb[d], uo[d] = own * u[d];
+    jbar_add(bb, uo);
   }
   // ---- u = M^-1 qf
-  float qfb[NV];
+  float qfb[NV], qfo[NV];
 #pragma unroll
   for (int d = 0; d < NV; ++d) qfb[d] = ub[d];
   ltl_solve<T>(t, qfb);
-  mbar_sub(qfb, u);
+#pragma unroll
+  for (int d = 0; d < NV; ++d) qfo[d] = own * qfb[d];
+  mbar_sub(qfo, u);
+  if constexpr (P == 2) {  // the pair's sums
+#pragma unroll
+    for (int k = 0; k < R::NMB; ++k) Mb[k] += pair_other(Mb[k]);
+#pragma unroll
+    for (int k = 0; k < 3 * NV; ++k) {
+      cab[k] += pair_other(cab[k]);
+      cvb[k] += pair_other(cvb[k]);
+    }
+#pragma unroll 1
+    for (int s = 0; s < n_c; ++s)
+#pragma unroll
+      for (int k = 0; k < 12; ++k) sb[12 * s + k] += pair_other(sb[12 * s + k]);
+  }
   // ---- actuators: qf[da] += clip(gain clip(act) + bias, force) gear
 #pragma unroll
   for (int a = 0; a < T::NU; ++a) {
@@ -410,6 +473,7 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
     if (c_in && f_in) gact[a] += qfb[da] * ac[AC_GEAR] * ac[AC_GAIN];
   }
 
+  if (P == 2 && sub != 0) return;  // lane 0 of the pair writes
 #pragma unroll
   for (int d = 0; d < NV; ++d) rec[R::QFB + d] = qfb[d];
 #pragma unroll

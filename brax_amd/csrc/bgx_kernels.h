@@ -207,12 +207,13 @@ __global__ __launch_bounds__(WAVE) void k_adj_fwd(AdjLaunch a) {
 }
 
 // adjoint stage A2: the reverse sweep from the tape; its row vectors live in
-// LDS, rev_lanes<T>() lanes per workgroup
+// LDS, rev_lanes<T>() lanes per workgroup, rev_pair<T>() lanes per env
 template <class T>
 __global__ __launch_bounds__(WAVE) void k_adj_rev(AdjLaunch a) {
-  constexpr int NQ = T::NQ, NV = T::NV, AL = rev_lanes<T>();
+  constexpr int NQ = T::NQ, NV = T::NV, AL = rev_lanes<T>(), P = rev_pair<T>();
   __shared__ float smem[RevLds<T>::SIZE * AL];
-  const int env = blockIdx.x * blockDim.x + threadIdx.x;
+  const long long lane = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int env = (int)(lane / P), sub = (int)(lane % P);
   if (env >= a.n) return;
   float q[NQ], qd[NV], gq1[NQ], gqd1[NV];
   float act[T::NU > 0 ? T::NU : 1], gact[T::NU > 0 ? T::NU : 1];
@@ -232,10 +233,12 @@ __global__ __launch_bounds__(WAVE) void k_adj_rev(AdjLaunch a) {
     act[k] = a.act ? a.act[(long long)env * T::NU + k] : 0.f;
     gact[k] = 0.f;
   }
-  adjoint_rev<T, AL>(a.M, q, qd, act, gq1, gqd1, RecPtr{a.rec + env, a.n}, smem + threadIdx.x,
-                     gact);
+  adjoint_rev<T, AL, P>(a.M, q, qd, act, gq1, gqd1, RecPtr{a.rec + env, a.n},
+                        smem + threadIdx.x, gact, sub);
+  if (sub == 0) {
 #pragma unroll
-  for (int k = 0; k < T::NU; ++k) a.gact[(long long)env * T::NU + k] += gact[k];
+    for (int k = 0; k < T::NU; ++k) a.gact[(long long)env * T::NU + k] += gact[k];
+  }
 }
 
 // adjoint stage B: one thread per (env, input coordinate), coordinate-major
@@ -291,8 +294,11 @@ struct Ops {
     return hipGetLastError();
   }
   static hipError_t adj_rev(const AdjLaunch& a, hipStream_t s) {
-    const int b = block_lanes(a.epw) < rev_lanes<T>() ? block_lanes(a.epw) : rev_lanes<T>();
-    hipLaunchKernelGGL(k_adj_rev<T>, env_grid(a.n, b), dim3(b), 0, s, a);
+    // whole lane pairs per workgroup
+    int b = block_lanes(a.epw) < rev_lanes<T>() ? block_lanes(a.epw) : rev_lanes<T>();
+    if (b < rev_pair<T>()) b = rev_pair<T>();
+    hipLaunchKernelGGL(k_adj_rev<T>, env_grid((long long)a.n * rev_pair<T>(), b), dim3(b), 0, s,
+                       a);
     return hipGetLastError();
   }
   static hipError_t adj_b(const AdjLaunch& a, hipStream_t s) {
