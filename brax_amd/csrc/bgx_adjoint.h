@@ -53,37 +53,26 @@ struct AdjRec : TapeLayout<T> {
   static constexpr int SIZE = GQDT + TL::NV;
 };
 
-// LDS of the reverse sweep per lane (floats; element e of a lane at
-// base[e * lanes], lanes = rev_lanes<T>()): row vectors over the NR
-// constraint rows, each slot's geometry (link, r, d0, d1, d2, diag) and
-// cotangents (d0, d1, d2, r), and the cdof cotangents
+// LDS of the reverse sweep.  Shared by the lanes of an env (element e of an
+// env at base[e * envs per workgroup]): the row vectors over the NR
+// constraint rows and each slot's geometry (link, r, d0, d1, d2, diag).
+// Private to a lane (element e at base[e * lanes per workgroup]): each
+// slot's cotangents (d0, d1, d2, r), the cdof cotangents, and the link
+// velocities and forces of the products in flight, indexed by link.
 template <class T>
 struct RevLds {
   using R = AdjRec<T>;
-  static constexpr int NR = R::NR, KS = R::KS, NV = T::NV;
+  static constexpr int NR = R::NR, KS = R::KS, NV = T::NV, L = T::L;
   static constexpr int XB = 0, YB = XB + NR, BB = YB + NR, DB = BB + NR, WB = DB + NR,
                        GB = WB + NR, RB = GB + NR, TR = RB + NR, TR2 = TR + NR;
-  static constexpr int SL = TR2 + NR;      // [KS][14]
-  static constexpr int SB = SL + 14 * KS;  // [KS][12]
+  static constexpr int SL = TR2 + NR;       // [KS][14]
+  static constexpr int SHARED = SL + 14 * KS;
+  static constexpr int SB = 0;              // [KS][12]
   static constexpr int CAB = SB + 12 * KS;  // [3 NV]
   static constexpr int CVB = CAB + 3 * NV;  // [3 NV]
-  static constexpr int SIZE = CVB + 3 * NV;
-};
-
-// lanes per workgroup of the reverse sweep: the most (a power of two up to
-// WAVE) whose LDS fits in 80 KB, so two workgroups share a CU
-template <class T>
-constexpr int rev_lanes() {
-  constexpr int b = RevLds<T>::SIZE * 4;
-  return b * 64 <= 81920 ? 64 : b * 32 <= 81920 ? 32 : b * 16 <= 81920 ? 16
-       : b * 8 <= 81920 ? 8 : b * 4 <= 81920 ? 4 : 1;
-}
-
-// a vector of LDS floats with a lane stride
-template <int S>
-struct LVec {
-  float* p;
-  BGX_D float& operator[](int r) const { return p[r * S]; }
+  static constexpr int LV = CVB + 3 * NV;   // [L][6] link velocity (ang, vel)
+  static constexpr int LF = LV + 6 * L;     // [L][6] link force (torque, force)
+  static constexpr int PRIVATE = LF + 6 * L;
 };
 
 // lanes per env of the reverse sweep: two lanes run the same instructions
@@ -92,6 +81,28 @@ template <class T>
 constexpr int rev_pair() {
   return 2;
 }
+
+// LDS floats of a reverse-sweep workgroup of a given lane count
+template <class T>
+constexpr int rev_lds_floats(int lanes) {
+  return RevLds<T>::SHARED * (lanes / rev_pair<T>()) + RevLds<T>::PRIVATE * lanes;
+}
+
+// lanes per workgroup of the reverse sweep: the most (a power of two up to
+// WAVE) whose LDS fits in 80 KB, so two workgroups share a CU
+template <class T>
+constexpr int rev_lanes() {
+  for (int b = 64; b > rev_pair<T>(); b /= 2)
+    if (rev_lds_floats<T>(b) * 4 <= 81920) return b;
+  return rev_pair<T>();
+}
+
+// a vector of LDS floats with a lane stride
+template <int S>
+struct LVec {
+  float* p;
+  BGX_D float& operator[](int r) const { return p[r * S]; }
+};
 
 // the value of the other lane of an env's lane pair
 BGX_D float pair_other(float v) { return __shfl_xor(v, 1); }
@@ -114,7 +125,7 @@ BGX_D V3 ld3(const RecPtr& rec, int f) { return v3(rec[f], rec[f + 1], rec[f + 2
 template <class T, int AL, int P = 1>
 BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float* qd,
                        const float* act, const float* gq1, const float* gqd1, RecPtr rec,
-                       float* lds, float* gact, int sub = 0) {
+                       float* shared, float* priv, float* gact, int sub = 0) {
   using R = AdjRec<T>;
   using Y = RevLds<T>;
   constexpr int NV = T::NV, L = T::L, NR = R::NR, NRC = R::NRC;
@@ -122,12 +133,17 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
   const int n_c = (int)rec[R::NC], iters = (int)rec[R::ITERS];
   // products both lanes form alike are counted by lane 0 only
   const float own = (P == 1 || sub == 0) ? 1.f : 0.f;
-  auto vec = [&](int off) { return LVec<AL>{lds + off * AL}; };
-  const LVec<AL> xb = vec(Y::XB), yb = vec(Y::YB), bb = vec(Y::BB), Db = vec(Y::DB),
-                 wb = vec(Y::WB), gb = vec(Y::GB), rb = vec(Y::RB), tr = vec(Y::TR),
-                 tr2 = vec(Y::TR2);
-  const LVec<AL> sl = vec(Y::SL), sb = vec(Y::SB), cab = vec(Y::CAB), cvb = vec(Y::CVB);
-  const LVec<AL> ay = gb;  // A r-bar; g-bar is dead by then
+  constexpr int ES = AL / P;  // envs per workgroup: stride of the shared region
+  using SV = LVec<ES>;
+  using PV = LVec<AL>;
+  auto vec = [&](int off) { return SV{shared + off * ES}; };
+  auto pvec = [&](int off) { return PV{priv + off * AL}; };
+  const SV xb = vec(Y::XB), yb = vec(Y::YB), bb = vec(Y::BB), Db = vec(Y::DB),
+           wb = vec(Y::WB), gb = vec(Y::GB), rb = vec(Y::RB), tr = vec(Y::TR),
+           tr2 = vec(Y::TR2), sl = vec(Y::SL);
+  const PV sb = pvec(Y::SB), cab = pvec(Y::CAB), cvb = pvec(Y::CVB), lv = pvec(Y::LV),
+           lf = pvec(Y::LF);
+  const SV ay = gb;  // A r-bar; g-bar is dead by then
 
   Tree<T> t;
   float lm_store[T::LM_LDS ? R::NMB : 1];
@@ -182,7 +198,7 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
     for (int i = 0; i < T::NLIM; ++i) f(NRC + i);
   };
   // tape row vector of iteration k into dst (4 loads in flight per slot)
-  auto load_row = [&](const LVec<AL>& dst, int base, int k) {
+  auto load_row = [&](const SV& dst, int base, int k) {
     const int b0 = base + k * NR;
 #pragma unroll 1
     for (int s = 0; s < n_c; ++s) {
@@ -197,17 +213,54 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
     for (int i = 0; i < T::NLIM; ++i) dst[NRC + i] = lv[i];
   };
 
-  // J z on the active rows
-  auto J_apply = [&](const float* z, const LVec<AL>& out) {
+  // link velocities under z into lv (links that host a contact)
+  auto store_link_vel = [&](const float* z) {
     V3 va[L], vv[L];
     link_vel<T>(t.ca, t.cv, z, va, vv);
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      if (hosts_contact<T>(l)) {
+        lv[6 * l] = va[l].x, lv[6 * l + 1] = va[l].y, lv[6 * l + 2] = va[l].z;
+        lv[6 * l + 3] = vv[l].x, lv[6 * l + 4] = vv[l].y, lv[6 * l + 5] = vv[l].z;
+      }
+    }
+  };
+  auto clear_link_force = [&]() {
+#pragma unroll
+    for (int l = 0; l < L; ++l)
+      if (hosts_contact<T>(l))
+#pragma unroll
+        for (int k = 0; k < 6; ++k) lf[6 * l + k] = 0.f;
+  };
+  auto add_link_force = [&](int l, V3 ta, V3 f) {
+    float* p = &lf[6 * l];
+    p[0] += ta.x, p[AL] += ta.y, p[2 * AL] += ta.z;
+    p[3 * AL] += f.x, p[4 * AL] += f.y, p[5 * AL] += f.z;
+  };
+  auto load_link_force = [&](V3* fa, V3* fv) {
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      if (hosts_contact<T>(l)) {
+        fa[l] = v3(lf[6 * l], lf[6 * l + 1], lf[6 * l + 2]);
+        fv[l] = v3(lf[6 * l + 3], lf[6 * l + 4], lf[6 * l + 5]);
+      } else {
+        fa[l] = v3zero(), fv[l] = v3zero();
+      }
+    }
+  };
+  auto slot_vel = [&](int s, V3& w, V3& v) {
+    const float* p = &lv[6 * (int)sl[14 * s]];
+    w = v3(p[0], p[AL], p[2 * AL]);
+    v = v3(p[3 * AL], p[4 * AL], p[5 * AL]);
+  };
+
+  // J z on the active rows
+  auto J_apply = [&](const float* z, const SV& out) {
+    store_link_vel(z);
 #pragma unroll 1
     for (int s = 0; s < n_c; ++s) {
-      const int l = (int)sl[14 * s];
-      V3 a = va[0], v = vv[0];
-#pragma unroll
-      for (int k = 1; k < L; ++k)
-        if (l == k) a = va[k], v = vv[k];
+      V3 a, v;
+      slot_vel(s, a, v);
       const V3 vp = v + cross(a, slot_v3(s, 1));
       const float e0 = dot(slot_v3(s, 4), vp);
       const float e1 = dot(slot_v3(s, 7), vp);
@@ -221,32 +274,28 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
     for (int i = 0; i < T::NLIM; ++i) out[NRC + i] = lside[i] * z[T::LIM_D[i]];
   };
   // slot s's pyramid rows of x as one force at the contact point
-  auto slot_force = [&](const LVec<AL>& x, int s, float& al0, float& al1, float& al2) {
+  auto slot_force = [&](const SV& x, int s, float& al0, float& al1, float& al2) {
     const float x0 = x[4 * s], x1 = x[4 * s + 1], x2 = x[4 * s + 2], x3 = x[4 * s + 3];
     al0 = (x0 + x1) + (x2 + x3), al1 = x0 - x1, al2 = x2 - x3;
     return slot_v3(s, 4) * al0 + slot_v3(s, 7) * al1 + slot_v3(s, 10) * al2;
   };
-  auto JT_apply = [&](const LVec<AL>& x, float* tau) {
-    V3 fa[L], fv[L];
-#pragma unroll
-    for (int l = 0; l < L; ++l) fa[l] = v3zero(), fv[l] = v3zero();
+  auto JT_apply = [&](const SV& x, float* tau) {
+    clear_link_force();
 #pragma unroll 1
     for (int s = 0; s < n_c; ++s) {
       float a0, a1, a2;
       const V3 w = slot_force(x, s, a0, a1, a2);
-      const V3 ta = cross(slot_v3(s, 1), w);
-      const int l = (int)sl[14 * s];
-#pragma unroll
-      for (int k = 0; k < L; ++k)
-        if (hosts_contact<T>(k) && l == k) fa[k] = fa[k] + ta, fv[k] = fv[k] + w;
+      add_link_force((int)sl[14 * s], cross(slot_v3(s, 1), w), w);
     }
+    V3 fa[L], fv[L];
+    load_link_force(fa, fv);
     project_forces<T>(t, fa, fv, tau);
 #pragma unroll
     for (int i = 0; i < T::NLIM; ++i) tau[T::LIM_D[i]] += lside[i] * x[NRC + i];
   };
   auto rdiag = [&](int r) { return r < NRC ? sl[14 * (r >> 2) + 13] : ldiag[r - NRC]; };
   // out = A x, ux = M^-1 J^T x
-  auto A_apply = [&](const LVec<AL>& x, const LVec<AL>& out, float* ux) {
+  auto A_apply = [&](const SV& x, const SV& out, float* ux) {
     JT_apply(x, ux);
     ltl_solve<T>(t, ux);
     J_apply(ux, out);
@@ -301,22 +350,16 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
   // the slot's link under c:  d_i-bar += alpha_i (v + w x r),
   // r-bar += f x w,  ca_d-bar += c_d (subtree sum of r x f),
   // cv_d-bar += c_d (subtree sum of f).
-  auto jbar_add = [&](const LVec<AL>& a, const float* c) {
-    V3 va[L], vv[L];
-    link_vel<T>(t.ca, t.cv, c, va, vv);
-    V3 fa[L], fv[L];
-#pragma unroll
-    for (int l = 0; l < L; ++l) fa[l] = v3zero(), fv[l] = v3zero();
+  auto jbar_add = [&](const SV& a, const float* c) {
+    store_link_vel(c);
+    clear_link_force();
 #pragma unroll 1
     for (int s = 0; s < n_c; ++s) {
       float a0, a1, a2;
       const V3 f = slot_force(a, s, a0, a1, a2);
       const V3 r = slot_v3(s, 1);
-      const int l = (int)sl[14 * s];
-      V3 w = va[0], v = vv[0];
-#pragma unroll
-      for (int k = 1; k < L; ++k)
-        if (l == k) w = va[k], v = vv[k];
+      V3 w, v;
+      slot_vel(s, w, v);
       const V3 vp = v + cross(w, r);
       const V3 rf = cross(f, w);
       float* g = &sb[12 * s];
@@ -324,11 +367,10 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
       g[3 * AL] += a1 * vp.x, g[4 * AL] += a1 * vp.y, g[5 * AL] += a1 * vp.z;
       g[6 * AL] += a2 * vp.x, g[7 * AL] += a2 * vp.y, g[8 * AL] += a2 * vp.z;
       g[9 * AL] += rf.x, g[10 * AL] += rf.y, g[11 * AL] += rf.z;
-      const V3 ta = cross(r, f);
-#pragma unroll
-      for (int k = 0; k < L; ++k)
-        if (hosts_contact<T>(k) && l == k) fa[k] = fa[k] + ta, fv[k] = fv[k] + f;
+      add_link_force((int)sl[14 * s], cross(r, f), f);
     }
+    V3 fa[L], fv[L];
+    load_link_force(fa, fv);
     V3 acc_a[L], acc_v[L];
 #pragma unroll
     for (int l = 0; l < L; ++l) acc_a[l] = v3zero(), acc_v[l] = v3zero();
@@ -718,9 +760,22 @@ BGX_D float adjoint_b_qd(const float* __restrict__ M, const float* q_in, const f
   return phi + rec[R::GQDT + j];
 }
 
+// one tree on a free root over a z-up plane: moving the root along x or y
+// moves everything rigidly, which no ingredient sees (frames about the root
+// COM, plane distances along z); only the position update passes through
+template <class T>
+constexpr bool xy_invariant() {
+  int roots = 0;
+  for (int l = 0; l < T::L; ++l) roots += T::PARENT[l] < 0 ? 1 : 0;
+  return T::PLANE_Z != 0 && roots == 1 && T::PARENT[0] < 0 && T::LTYPE[0] == 0 &&
+         T::QADR[0] == 0;
+}
+
 template <class T>
 BGX_D float adjoint_b(const float* __restrict__ M, const float* q_in, const float* qd_in,
                       const float* act, RecPtr rec, int dir) {
+  if constexpr (xy_invariant<T>())
+    if (dir < 2) return rec[AdjRec<T>::GQ1 + dir];
   return dir < T::NQ ? adjoint_b_q<T>(M, q_in, qd_in, act, rec, dir)
                      : adjoint_b_qd<T>(M, q_in, qd_in, act, rec, dir - T::NQ);
 }

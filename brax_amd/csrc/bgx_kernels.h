@@ -211,7 +211,7 @@ __global__ __launch_bounds__(WAVE) void k_adj_fwd(AdjLaunch a) {
 template <class T>
 __global__ __launch_bounds__(WAVE) void k_adj_rev(AdjLaunch a) {
   constexpr int NQ = T::NQ, NV = T::NV, AL = rev_lanes<T>(), P = rev_pair<T>();
-  __shared__ float smem[RevLds<T>::SIZE * AL];
+  __shared__ float smem[rev_lds_floats<T>(AL)];
   const long long lane = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int env = (int)(lane / P), sub = (int)(lane % P);
   if (env >= a.n) return;
@@ -234,7 +234,8 @@ __global__ __launch_bounds__(WAVE) void k_adj_rev(AdjLaunch a) {
     gact[k] = 0.f;
   }
   adjoint_rev<T, AL, P>(a.M, q, qd, act, gq1, gqd1, RecPtr{a.rec + env, a.n},
-                        smem + threadIdx.x, gact, sub);
+                        smem + threadIdx.x / P,
+                        smem + RevLds<T>::SHARED * (AL / P) + threadIdx.x, gact, sub);
   if (sub == 0) {
 #pragma unroll
     for (int k = 0; k < T::NU; ++k) a.gact[(long long)env * T::NU + k] += gact[k];
@@ -294,9 +295,8 @@ struct Ops {
     return hipGetLastError();
   }
   static hipError_t adj_rev(const AdjLaunch& a, hipStream_t s) {
-    // whole lane pairs per workgroup
-    int b = block_lanes(a.epw) < rev_lanes<T>() ? block_lanes(a.epw) : rev_lanes<T>();
-    if (b < rev_pair<T>()) b = rev_pair<T>();
+    // rev_lanes<T>() lanes per workgroup: the LDS layout's strides
+    const int b = rev_lanes<T>();
     hipLaunchKernelGGL(k_adj_rev<T>, env_grid((long long)a.n * rev_pair<T>(), b), dim3(b), 0, s,
                        a);
     return hipGetLastError();
