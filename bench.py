@@ -99,29 +99,27 @@ def measured_traffic(name, n, steps):
   return None
 
 
-def cpu_baseline(sys_, q0, qd0, act_range, rng_seed=1, n=256, steps=8):
-  """Oracle (float32, torch CPU) on the first n envs for `steps` env steps:
+def cpu_baseline(sys_, q0, qd0, act_range, rng_seed=1, n=4096, steps=100):
+  """The oracle's C++ restatement (oracle/cstep.cpp, float32, OpenMP over
+  envs on up to 16 host threads) on the first n envs for `steps` env steps:
   env-steps/s on this host, and the mean active constraint-row count."""
-  import torch
-  from oracle.generalized import Oracle
-  threads = torch.get_num_threads()
-  o = Oracle(sys_, dtype=torch.float32, inverse="exact")
+  import os
+  from oracle.cstep import CStep
+  threads = min(os.cpu_count() or 1, 16)
+  n = min(n, q0.shape[0])
+  cs = CStep(sys_)
   rng = np.random.default_rng(rng_seed)
   acts = rng.uniform(-act_range, act_range, (steps, n, sys_.act_size()))
-  st = o.init(q0[:n].astype(np.float64), qd0[:n].astype(np.float64))
-  rows = []
+  cs.rollout(q0[:4], qd0[:4], acts[:1, :4], N_FRAMES, threads=1)  # load, warm
   t0 = time.perf_counter()
-  for s in range(steps):
-    for _ in range(N_FRAMES):
-      st = o.step(st, acts[s])
-      rows.append((st.con_jac.abs().sum(-1) > 0).sum(-1).float().mean().item())
+  _, _, rows = cs.rollout(q0[:n], qd0[:n], acts, N_FRAMES, threads=threads, rows=True)
   dt = time.perf_counter() - t0
   return {
       "value": n * steps / dt, "unit": "env-steps/s", "cores": threads,
       "kind": "port",
       "sample": f"{n} {sys_.name} envs x {steps} env steps ({N_FRAMES} substeps), "
-                f"oracle/generalized.py float32 (torch CPU), {dt:.1f} s",
-  }, float(np.mean(rows))
+                f"oracle/cstep.cpp float32, OpenMP {threads} threads, {dt:.1f} s",
+  }, rows
 
 
 def forward_bench(args, name):
@@ -203,8 +201,9 @@ def forward_bench(args, name):
   cpu = None
   r_a = 9.9
   if not args.no_cpu:
-    cpu, r_a = cpu_baseline(sys_, q0, qd0, act_range, n=256 if name == "quadruped" else 64,
-                            steps=64 if name == "quadruped" else 48)
+    cpu, r_a = cpu_baseline(sys_, q0, qd0, act_range,
+                            n=4096 if name == "quadruped" else 2048,
+                            steps=200 if name == "quadruped" else 100)
   nv, L = sys_.qd_size(), sys_.num_links()
   fl = flop_per_substep(L, nv, sys_.num_contacts(), sys_.num_limit_rows(),
                         len(sys_.geom_type), sys_.solver_iterations, r_a)
