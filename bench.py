@@ -195,7 +195,9 @@ def forward_bench(args, name):
     del src, dst
 
   st = status.cpu().numpy()
-  finite = bool(torch.isfinite(q).all() and torch.isfinite(qd).all())
+  fin_env = torch.isfinite(q).all(1) & torch.isfinite(qd).all(1)
+  finite = bool(fin_env.all())
+  nonfinite_envs = int((~fin_env).sum())
   env_steps_per_s = n * K / wall
 
   cpu = None
@@ -240,6 +242,7 @@ def forward_bench(args, name):
       "cpu_baseline": cpu,
       "slow_path_envs": int((st & 1).sum()),
       "finite": finite,
+      "nonfinite_envs": nonfinite_envs,
   }
 
 
@@ -389,7 +392,9 @@ def hetero_bench(args):
   torch.cuda.synchronize()
   wall = time.perf_counter() - t0
   kernel_ms = ev0.elapsed_time(ev1)
-  finite = all(bool(torch.isfinite(x).all()) for x in q + qd)
+  nonfinite_envs = sum(int((~(torch.isfinite(a).all(1) & torch.isfinite(b).all(1))).sum())
+                       for a, b in zip(q, qd))
+  finite = nonfinite_envs == 0
   fl = 0.0
   for g in hb.groups:
     s0 = envs[g.envs[0]]
@@ -422,6 +427,7 @@ def hetero_bench(args):
                    "note": "quadruped rows at the 65536-env mean 9.9"},
       "cpu_baseline": None,
       "finite": finite,
+      "nonfinite_envs": nonfinite_envs,
   }
 
 
