@@ -261,7 +261,7 @@ constexpr int lds_floats() {
 // code size bounded (the factor then lives in scratch memory)
 template <class T>
 constexpr int unroll_nv() {
-  return T::NV <= 20 ? 32 : 1;
+  return T::NV <= 24 ? 32 : 1;
 }
 
 // M = L^T L in place on the lower part of M (Featherstone's LTL: leaves
