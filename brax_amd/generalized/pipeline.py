@@ -165,11 +165,37 @@ def _act(m: DeviceModel, act, n: int):
   return a
 
 
+def _needs_grad(*ts) -> bool:
+  return torch.is_grad_enabled() and any(
+      t is not None and torch.is_tensor(t) and t.requires_grad for t in ts)
+
+
 def pipeline_step(sys: S.System, state: State, act, n_frames: int = 1,
                   debug: bool = False) -> State:
-  """``n_frames`` substeps with one action (brax/envs/base.py:128-137)."""
+  """``n_frames`` substeps with one action (brax/envs/base.py:128-137).
+
+  Differentiable like the reference's step under jax.grad: when q, qd or
+  act require gradients, the new q and qd carry them through the
+  hand-written adjoint (a one-step ``rollout_grad``); the frames x, xd of
+  the returned State do not.
+  """
   _debug_unsupported(debug)
   m = device_model(sys, state.q.device)
+  if _needs_grad(state.q, state.qd, act):
+    n = state.q.shape[0]
+    if m.nu:
+      a = torch.as_tensor(act, dtype=torch.float32, device=m.device)
+      a = (a[None] if a.dim() == 1 else a).expand(n, m.nu)[None]
+    else:
+      a = torch.zeros(1, n, 0, device=m.device)
+    qt, qdt, status = rollout_grad(sys, state.q, state.qd, a, n_frames, ckpt_every=1)
+    q, qd = qt[0], qdt[0]
+    xp, xr, xa, xv = _frames(m, n, m.device)
+    lib = _lib.load()
+    qc, qdc = q.detach().contiguous(), qd.detach().contiguous()
+    _lib.check(lib.bgx_init(m.handle, n, _lib.ptr(qc), _lib.ptr(qdc), _lib.ptr(xp),
+                            _lib.ptr(xr), _lib.ptr(xa), _lib.ptr(xv), _stream()))
+    return State(q=q, qd=qd, x=Transform(xp, xr), xd=Motion(xa, xv), status=status)
   q = state.q.contiguous().clone()
   qd = state.qd.contiguous().clone()
   n = q.shape[0]

@@ -264,3 +264,27 @@ def test_adjoint_vs_float64_central_differences():
     print(f"\nFD eps {eps:g}: median rel err {np.median(rel):.2e}, "
           f"within 1e-3: {(rel < 1e-3).mean():.2f}", flush=True)
   assert (stats[1e-6] < 1e-3).mean() >= 0.9
+
+
+def test_pipeline_step_is_differentiable():
+  """pipeline_step carries gradients through the adjoint when its inputs
+  require them: the same gradients as a one-step rollout_grad, and the
+  same new state as the plain step."""
+  sys = S.quadruped(0)
+  q0, qd0, acts, wq, wqd = _inputs(sys, 32, 1, seed=12)
+  q = torch.tensor(q0, device=DEV, requires_grad=True)
+  qd = torch.tensor(qd0, device=DEV, requires_grad=True)
+  a = torch.tensor(acts[0], device=DEV, requires_grad=True)
+  st = pipeline.pipeline_step(sys, pipeline.State(q=q, qd=qd, x=None, xd=None), a, 5)
+  ((torch.tensor(wq[0], device=DEV) * st.q).sum() +
+   (torch.tensor(wqd[0], device=DEV) * st.qd).sum()).backward()
+  ref = _gpu_grads(sys, q0, qd0, acts, wq, wqd, 5, 1)
+  torch.cuda.synchronize()
+  np.testing.assert_array_equal(q.grad.cpu().numpy(), ref[0])
+  np.testing.assert_array_equal(qd.grad.cpu().numpy(), ref[1])
+  np.testing.assert_array_equal(a.grad.cpu().numpy(), ref[2][0])
+  with torch.no_grad():
+    plain = pipeline.pipeline_step(sys, pipeline.State(q=q.detach(), qd=qd.detach(), x=None,
+                                                       xd=None), a.detach(), 5)
+  np.testing.assert_array_equal(st.q.detach().cpu().numpy(), plain.q.cpu().numpy())
+  np.testing.assert_array_equal(st.x.pos.cpu().numpy(), plain.x.pos.cpu().numpy())
