@@ -8,10 +8,10 @@
 // derivative, and the smooth arithmetic between them is differentiated.
 // This file computes the same derivative in two stages:
 //
-//  A1 (one thread per env): the step's own substep (substep_impl, a slot per
-//    candidate) recording its tape: tree, contact slots, accelerations and
-//    every projected-gradient iterate.
-//  A2 (one thread per env): the reverse sweep of the integrator, the solves
+//  A1 (one thread per env and substep of a segment): the step's own substep
+//    (substep_impl, a slot per candidate) recording its tape: tree, contact
+//    slots, accelerations and every projected-gradient iterate.
+//  A2 (two lanes per env): the reverse sweep of the integrator, the solves
 //    with M = L^T L and the unrolled FISTA iterations, by hand, from the
 //    tape.  Output: cotangents of the position- and velocity-dependent
 //    ingredients -- qf_smooth, the mass matrix (a symmetric matrix on the
@@ -21,7 +21,7 @@
 //    through products a^T J c, and each such product is carried back to the
 //    quantities J is built from (cdof, the slots' lever arms and pyramid
 //    directions), a few floats per dof and per slot.  The row vectors of the
-//    sweep live in LDS.
+//    sweep live in LDS; the lane pair splits each iteration's products.
 //  B (one thread per env and input coordinate): run the same formulas that
 //    build those ingredients from (q, qd) in dual numbers along one input
 //    coordinate and contract the tangents with the cotangents of stage A:
