@@ -16,6 +16,13 @@ forms.  Files:
   humanoid.npz   synthetic humanoid, q noise U(-0.01, 0.01) with the pelvis
                  lowered by U(0, 0.03) m (feet in contact), qd U(-0.01, 0.01),
                  act U(-0.4, 0.4); 3 env steps (float64), 1 step (float32)
+  chains.npz     hinge chains of 2, 7, 16 and 32 links, 4 envs each with
+                 their own model (seeds 0-3), q U(-0.3, 0.3), qd 0.3 N(0,1),
+                 act U(-1, 1); 20 env steps (float64)
+  stages.npz     the quadruped's first substep from the golden inputs (first
+                 4 envs), stage by stage: frames x, xd, root COM, cdof, the
+                 mass matrix, constraint rows J, d, aref, qf_smooth, the
+                 solver's force and qf_constraint (float64)
 """
 
 import os
@@ -113,9 +120,46 @@ def humanoid(path):
                       q1_f32=st32.q.double().numpy(), qd1_f32=st32.qd.double().numpy())
 
 
+def chains(path):
+  out = {}
+  for L in (2, 7, 16, 32):
+    rng = np.random.default_rng(L)
+    n, steps = 4, 20
+    q0 = rng.uniform(-0.3, 0.3, (n, L))
+    qd0 = 0.3 * rng.standard_normal((n, L))
+    acts = rng.uniform(-1, 1, (steps, n, L))
+    qs, qds = [], []
+    for e in range(n):
+      q, qd = _trajectory(Oracle(S.chain(L, e), F64, inverse="exact"), q0[e:e + 1],
+                          qd0[e:e + 1], acts[:, e:e + 1], 5)
+      qs.append(q[:, 0]), qds.append(qd[:, 0])
+    out.update({f"q0_{L}": q0, f"qd0_{L}": qd0, f"acts_{L}": acts,
+                f"q_traj_{L}": np.stack(qs, 1), f"qd_traj_{L}": np.stack(qds, 1)})
+  np.savez_compressed(path, **out)
+
+
+def stages(path):
+  sys = S.quadruped(0)
+  g = np.load(os.path.join(HERE, "quadruped.npz"))
+  q0, qd0, act = g["q0"][:4], g["qd0"][:4], g["acts"][0, :4]
+  o = Oracle(sys, F64, inverse="exact")
+  st0 = o.init(q0, qd0)
+  st1 = o.step(st0, act)
+  np.savez_compressed(
+      path, q0=q0, qd0=qd0, act=act, x_pos=st0.x_pos.numpy(), x_rot=st0.x_rot.numpy(),
+      xd_ang=st0.xd_ang.numpy(), xd_vel=st0.xd_vel.numpy(), root_com=st0.root_com.numpy(),
+      cdof_ang=st0.cdof[0].numpy(), cdof_vel=st0.cdof[1].numpy(),
+      mass_mx=st0.mass_mx.numpy(), con_jac=st0.con_jac.numpy(),
+      con_diag=st0.con_diag.numpy(), con_aref=st0.con_aref.numpy(),
+      qf_smooth=st1.qf_smooth.numpy(), force=st1.pg_info["force"].numpy(),
+      qf_constraint=st1.qf_constraint.numpy(), q1=st1.q.numpy(), qd1=st1.qd.numpy())
+
+
 if __name__ == "__main__":
   quadruped(os.path.join(HERE, "quadruped.npz"))
   pendulum3(os.path.join(HERE, "pendulum3.npz"))
   ant(os.path.join(HERE, "ant.npz"))
   humanoid(os.path.join(HERE, "humanoid.npz"))
+  chains(os.path.join(HERE, "chains.npz"))
+  stages(os.path.join(HERE, "stages.npz"))
   print("written")

@@ -105,6 +105,26 @@ def test_long_chain_teacher_forced(n_links):
   assert (g <= np.maximum(1.0, 20.0 * f)).all()
 
 
+@pytest.mark.parametrize("n_links", [2, 7])
+def test_chain_golden_20_steps(n_links):
+  """The committed chain golden vectors (float64 oracle, 4 envs with their
+  own models): 20 env steps free-running within 1e-4 relative (1e-5
+  absolute)."""
+  import os
+  g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "chains.npz"))
+  L = n_links
+  batch = pipeline.SystemBatch([S.chain(L, e) for e in range(4)])
+  st = pipeline.init(batch, torch.tensor(g[f"q0_{L}"], dtype=torch.float32, device=DEV),
+                     torch.tensor(g[f"qd0_{L}"], dtype=torch.float32, device=DEV))
+  _, (qt, qdt) = pipeline.rollout(batch, st, torch.tensor(g[f"acts_{L}"], dtype=torch.float32,
+                                                          device=DEV), 5, trajectory=True)
+  torch.cuda.synchronize()
+  worst = max(_ratio(qt.cpu().numpy(), g[f"q_traj_{L}"]).max(),
+              _ratio(qdt.cpu().numpy(), g[f"qd_traj_{L}"]).max())
+  print(f"\nchain{L} golden: worst tolerance ratio over 20 steps {worst:.3f}", flush=True)
+  assert worst <= 1.0
+
+
 def test_hetero_batch_matches_homogeneous_runs():
   rng = np.random.default_rng(11)
   quad = S.quadruped(0)
