@@ -42,6 +42,7 @@ SIGNATURES = {
                          c_vp, c_vp, c_vp, c_vp]),
     "bgx_rollout": (c_int, [c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_i64,
                             c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "bgx_contacts": (c_int, [c_vp, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "bgx_step_probe": (c_int, [c_vp, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp,
                                c_vp]),
     "bgx_adjoint_workspace_bytes": (c_i64, [c_vp, c_i32, c_i32, c_i32]),

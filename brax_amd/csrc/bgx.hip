@@ -187,6 +187,23 @@ int bgx_rollout(const bgx_model* m, int32_t n, int32_t n_steps, int32_t n_frames
   return launch(m, false, a, stream);
 }
 
+int bgx_contacts(const bgx_model* m, int32_t n, const float* q, float* dist, float* pos,
+                 float* frame, void* stream) {
+  if (!m) return fail(BGX_ERR_ARG, "model is null");
+  if (!valid_topo(m->topo)) return fail(BGX_ERR_TOPOLOGY, "unknown topology id");
+  if (n < 0) return fail(BGX_ERR_ARG, "negative env count");
+  const bgx::TopoOps* ops = kOps[m->topo];
+  if (n == 0 || !ops->contacts) return BGX_OK;  // no contact candidates
+  if (!q || !dist || !pos || !frame) return fail(BGX_ERR_ARG, "null argument");
+  if (ops->per_env && n > m->n_tables)
+    return fail(BGX_ERR_ARG, "more envs than per-env model tables");
+  bgx::Launch a{};
+  a.M = m->table, a.n = n, a.epw = m->epw[BGX_KERNELS_STEP];
+  a.q = const_cast<float*>(q);
+  a.c_dist = dist, a.c_pos = pos, a.c_frame = frame;
+  return check_hip(ops->contacts(a, static_cast<hipStream_t>(stream)), "kernel launch");
+}
+
 int bgx_step_probe(const bgx_model* m, int32_t n, int32_t n_frames, float* q, float* qd,
                    const float* act, float* probe, void* stream) {
   if (!probe) return fail(BGX_ERR_ARG, "probe is null");

@@ -44,6 +44,11 @@ struct Launch {
   // n_frames n NQ (sub_qd alike), status bits ORed atomically
   int n_seg;
   long long seg_act_stride;
+  // contacts of the state (bgx_contacts): dist (n, NCAND), pos (n, NCAND, 3),
+  // frame (n, NCAND, 3, 3) rows n, t1, t2
+  float* c_dist;
+  float* c_pos;
+  float* c_frame;
 };
 
 // one adjoint substep: state (q, qd) of substep i, its action, the running
@@ -115,6 +120,38 @@ __global__ __launch_bounds__(WAVE) void k_init(Launch a) {
 #pragma unroll
   for (int k = 0; k < T::NV; ++k) qd[k] = a.qd[(long long)env * T::NV + k];
   write_frames<T>(a, env, q, qd);
+}
+
+// every contact candidate of the state (the reference's debug contact
+// output, brax/generalized/pipeline.py:58-59): MJX plane collisions at q
+template <class T>
+__global__ __launch_bounds__(WAVE) void k_contacts(Launch a) {
+  const int env = blockIdx.x * blockDim.x + threadIdx.x;
+  if (env >= a.n) return;
+  float q[T::NQ], qd[T::NV];
+#pragma unroll
+  for (int k = 0; k < T::NQ; ++k) q[k] = a.q[(long long)env * T::NQ + k];
+#pragma unroll
+  for (int k = 0; k < T::NV; ++k) qd[k] = 0.f;
+  const float* M = env_model<T>(a.M, env);
+  V3 xp[T::L];
+  Q4 xr[T::L];
+  forward_kinematics<T, false, float>(M, q, qd, xp, xr, nullptr, nullptr);
+#pragma unroll
+  for (int c = 0; c < T::NCAND; ++c) {
+    V3 ctr, axis, pos, t1, t2;
+    candidate_center<T>(M, c, xp, xr, ctr, axis);
+    const float dist = candidate_dist<T>(M, c, ctr);
+    contact_frame<T>(M, c, ctr, axis, dist, -1, pos, t1, t2);
+    const V3 n = T::PLANE_Z ? v3(0.f, 0.f, 1.f) : ldv3(cand_rec<T>(M, c) + CD_PN);
+    const long long o = (long long)env * T::NCAND + c;
+    a.c_dist[o] = dist;
+    a.c_pos[3 * o] = pos.x, a.c_pos[3 * o + 1] = pos.y, a.c_pos[3 * o + 2] = pos.z;
+    float* f = a.c_frame + 9 * o;
+    f[0] = n.x, f[1] = n.y, f[2] = n.z;
+    f[3] = t1.x, f[4] = t1.y, f[5] = t1.z;
+    f[6] = t2.x, f[7] = t2.y, f[8] = t2.z;
+  }
 }
 
 template <class T>
@@ -272,6 +309,7 @@ struct TopoOps {
   hipError_t (*adj_fwd)(const AdjLaunch&, hipStream_t);
   hipError_t (*adj_rev)(const AdjLaunch&, hipStream_t);
   hipError_t (*adj_b)(const AdjLaunch&, hipStream_t);
+  hipError_t (*contacts)(const Launch&, hipStream_t);
 };
 
 template <class T>
@@ -307,6 +345,11 @@ struct Ops {
                        0, s, a);
     return hipGetLastError();
   }
+  static hipError_t contacts(const Launch& a, hipStream_t s) {
+    const int b = block_lanes(a.epw);
+    hipLaunchKernelGGL(k_contacts<T>, env_grid(a.n, b), dim3(b), 0, s, a);
+    return hipGetLastError();
+  }
   static TopoOps table() {
     TopoOps t{T::NAME,
               T::SIG,
@@ -317,7 +360,9 @@ struct Ops {
               &rollout,
               nullptr,
               nullptr,
+              nullptr,
               nullptr};
+    if constexpr (T::NCAND > 0) t.contacts = &contacts;
     // the adjoint is compiled for topologies with a shared model table
     if constexpr (!T::PER_ENV) t.adj_fwd = &adj_fwd, t.adj_rev = &adj_rev, t.adj_b = &adj_b;
     return t;

@@ -29,6 +29,22 @@ class Motion:
 
 
 @dataclasses.dataclass
+class Contact:
+  """Contacts of a state (brax/base.py Contact; the generalized pipeline's
+  debug output): every candidate of the topology, active (dist < 0) or not.
+  pos (N, C, 3), frame (N, C, 3, 3) rows normal, t1, t2, dist (N, C);
+  friction (C,) the tangential coefficient, solref (C, 2), solimp (C, 5);
+  link_idx = (link of the plane, link of the geom), each (C,) (-1 = world)."""
+  pos: torch.Tensor
+  frame: torch.Tensor
+  dist: torch.Tensor
+  friction: torch.Tensor
+  solref: torch.Tensor
+  solimp: torch.Tensor
+  link_idx: tuple
+
+
+@dataclasses.dataclass
 class State:
   q: torch.Tensor
   qd: torch.Tensor

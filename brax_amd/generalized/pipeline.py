@@ -24,7 +24,7 @@ import torch
 from brax_amd import _lib
 from brax_amd import codegen
 from brax_amd import systems as S
-from brax_amd.generalized.base import Motion, State, Transform
+from brax_amd.generalized.base import Contact, Motion, State, Transform
 
 STATUS_SLOW_PATH = 1
 STATUS_NONFINITE = 2
@@ -131,16 +131,44 @@ def _frames(m: DeviceModel, n: int, device):
           torch.empty(n, m.L, 3, device=device))
 
 
-def _debug_unsupported(debug: bool):
-  if debug:
-    raise NotImplementedError("debug contact output is not produced by this step")
+def contacts(sys: S.System, q) -> Optional[Contact]:
+  """The contacts of states q (N, q_size): what the reference's pipeline
+  attaches to its State with debug=True (brax/generalized/pipeline.py:58-59,
+  91-92), from the GPU's plane collisions; None for a topology without
+  contact candidates."""
+  m = device_model(sys, q.device if torch.is_tensor(q) and q.is_cuda else None)
+  qb = _as_batch(q, m.nq, m.device, "q").detach()
+  n = qb.shape[0]
+  topo = codegen.Topology(sys.systems[0] if isinstance(sys, SystemBatch) else sys,
+                          per_env=isinstance(sys, SystemBatch))
+  c = topo.ncand
+  if c == 0:
+    return None
+  dist = torch.empty(n, c, device=m.device)
+  pos = torch.empty(n, c, 3, device=m.device)
+  frame = torch.empty(n, c, 3, 3, device=m.device)
+  lib = _lib.load()
+  _lib.check(lib.bgx_contacts(m.handle, n, _lib.ptr(qb), _lib.ptr(dist), _lib.ptr(pos),
+                              _lib.ptr(frame), _stream()))
+  srcs = [p for p, _ in topo.cand_src]
+  t = lambda x: torch.tensor(np.asarray(x, np.float32), device=m.device)  # noqa: E731
+  first = sys.systems[0] if isinstance(sys, SystemBatch) else sys
+  la = [int(first.geom_bodyid[p.geom1]) - 1 for p in srcs]
+  lb = [int(first.geom_bodyid[p.geom2]) - 1 for p in srcs]
+  return Contact(pos=pos, frame=frame, dist=dist,
+                 friction=t([p.friction[0] for p in srcs]),
+                 solref=t([p.solref for p in srcs]), solimp=t([p.solimp for p in srcs]),
+                 link_idx=(torch.tensor(la, device=m.device), torch.tensor(lb, device=m.device)))
+
+
+def _with_contacts(sys, state: State, debug: bool) -> State:
+  return state.replace(contact=contacts(sys, state.q)) if debug else state
 
 
 def init(sys: S.System, q, qd, unused_act=None, unused_ctrl=None,
          debug: bool = False, device=None) -> State:
   """pipeline.init (brax/generalized/pipeline.py:32-61)."""
   del unused_act, unused_ctrl
-  _debug_unsupported(debug)
   if device is None:
     device = q.device if torch.is_tensor(q) and q.is_cuda else "cuda"
   m = device_model(sys, device)
@@ -151,7 +179,8 @@ def init(sys: S.System, q, qd, unused_act=None, unused_ctrl=None,
   lib = _lib.load()
   _lib.check(lib.bgx_init(m.handle, n, _lib.ptr(qb), _lib.ptr(qdb), _lib.ptr(xp),
                           _lib.ptr(xr), _lib.ptr(xa), _lib.ptr(xv), _stream()))
-  return State(q=qb, qd=qdb, x=Transform(xp, xr), xd=Motion(xa, xv))
+  return _with_contacts(sys, State(q=qb, qd=qdb, x=Transform(xp, xr), xd=Motion(xa, xv)),
+                        debug)
 
 
 def _act(m: DeviceModel, act, n: int):
@@ -177,9 +206,9 @@ def pipeline_step(sys: S.System, state: State, act, n_frames: int = 1,
   Differentiable like the reference's step under jax.grad: when q, qd or
   act require gradients, the new q and qd carry them through the
   hand-written adjoint (a one-step ``rollout_grad``); the frames x, xd of
-  the returned State do not.
+  the returned State do not.  With debug=True the State carries the
+  contacts of the new state (``contacts``).
   """
-  _debug_unsupported(debug)
   m = device_model(sys, state.q.device)
   if _needs_grad(state.q, state.qd, act):
     n = state.q.shape[0]
@@ -195,7 +224,8 @@ def pipeline_step(sys: S.System, state: State, act, n_frames: int = 1,
     qc, qdc = q.detach().contiguous(), qd.detach().contiguous()
     _lib.check(lib.bgx_init(m.handle, n, _lib.ptr(qc), _lib.ptr(qdc), _lib.ptr(xp),
                             _lib.ptr(xr), _lib.ptr(xa), _lib.ptr(xv), _stream()))
-    return State(q=q, qd=qd, x=Transform(xp, xr), xd=Motion(xa, xv), status=status)
+    return _with_contacts(sys, State(q=q, qd=qd, x=Transform(xp, xr), xd=Motion(xa, xv),
+                                     status=status), debug)
   q = state.q.contiguous().clone()
   qd = state.qd.contiguous().clone()
   n = q.shape[0]
@@ -206,8 +236,8 @@ def pipeline_step(sys: S.System, state: State, act, n_frames: int = 1,
   _lib.check(lib.bgx_step(m.handle, n, int(n_frames), _lib.ptr(q), _lib.ptr(qd),
                           _lib.ptr(a), _lib.ptr(xp), _lib.ptr(xr), _lib.ptr(xa),
                           _lib.ptr(xv), _lib.ptr(status), _stream()))
-  return State(q=q, qd=qd, x=Transform(xp, xr), xd=Motion(xa, xv),
-               status=status)
+  return _with_contacts(sys, State(q=q, qd=qd, x=Transform(xp, xr), xd=Motion(xa, xv),
+                                   status=status), debug)
 
 
 def step(sys: S.System, state: State, act, debug: bool = False) -> State:

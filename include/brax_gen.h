@@ -114,6 +114,14 @@ int bgx_rollout(const bgx_model* model, int32_t n_envs, int32_t n_steps,
                 float* x_pos, float* x_rot, float* xd_ang, float* xd_vel,
                 int32_t* status, void* stream);
 
+/* Every contact candidate of the states q (N, NQ), as the reference's
+ * debug contact output (brax/generalized/pipeline.py:58-59, 91-92; MJX plane
+ * collisions, brax/contact.py:28-67): dist (N, C), pos (N, C, 3) and frame
+ * (N, C, 3, 3) with rows normal, t1, t2, for the C = dims[5] candidates in
+ * the topology's order.  A topology without candidates writes nothing. */
+int bgx_contacts(const bgx_model* model, int32_t n_envs, const float* q, float* dist,
+                 float* pos, float* frame, void* stream);
+
 /* Diagnostic variant of bgx_step: probe (N, 2 * NV + 3) receives, for the
  * last substep, qf_smooth (dynamics.forward), qdd (integrator), and the
  * active contact count, solver iterations and line-search trials. */
