@@ -1094,3 +1094,172 @@ def from_arrays_check(sys: System) -> None:
     if t not in Q_WIDTHS:
       raise NotImplementedError(f"link type {t!r}")
   _ = sys.contact_pairs()
+
+
+# ---------------------------------------------------------------------------
+# MJCF subset from XML (brax.io.mjcf.loads / load_model, brax/io/mjcf.py:317-512)
+
+_MJCF_GEOMS = {"plane": GEOM_PLANE, "sphere": GEOM_SPHERE, "capsule": GEOM_CAPSULE}
+_MJCF_IGNORED = {"camera", "light", "site"}
+# attributes with no effect on the generalized step: names, visuals, and
+# fields the step does not read (a free joint's margin, geom user data)
+_MJCF_VISUAL = {"name", "rgba", "material", "user", "group"}
+_MJCF_ATTRS = {
+    "geom": {"type", "size", "pos", "quat", "fromto", "density", "mass", "contype",
+             "conaffinity", "friction", "condim", "solref", "solimp"},
+    "joint": {"type", "axis", "pos", "range", "limited", "armature", "damping",
+              "stiffness", "solreflimit", "solimplimit", "margin"},
+    "body": {"pos", "quat", "childclass"},
+    "motor": {"joint", "gear", "ctrlrange", "ctrllimited"},
+}
+
+
+def _mjcf_check(tag: str, a: dict) -> None:
+  extra = set(a) - _MJCF_ATTRS[tag] - _MJCF_VISUAL
+  if extra:
+    raise NotImplementedError(f"<{tag}> attributes {sorted(extra)}")
+  if a.get("condim", "3") != "3":
+    raise NotImplementedError("condim other than 3")
+  if "childclass" in a:
+    raise NotImplementedError("default classes")
+  if tag == "joint" and "margin" in a and a.get("type", "hinge") != "free":
+    raise NotImplementedError("joint limit margin")
+
+
+def _floats(s) -> List[float]:
+  return [float(v) for v in str(s).split()]
+
+
+def load_mjcf(xml: str, name: Optional[str] = None) -> System:
+  """Compiles MJCF text into a System: the subset the step supports.
+
+  Read: <compiler angle>, <option timestep iterations gravity>, brax's
+  <custom> numerics init_qpos, solver_maxls, matrix_inv_iterations
+  (defaults 20 and 10 as mjcf.py:154-172; the other numerics belong to the
+  positional and spring pipelines), the unnamed top-level <default> of
+  joint / geom / motor, <worldbody> bodies (pos, quat) with free or hinge
+  joints (axis, pos, range, limited, armature, damping, stiffness,
+  solreflimit, solimplimit), plane / sphere / capsule geoms (size, pos,
+  quat, fromto, density, mass, contype, conaffinity, friction, solref,
+  solimp) and <actuator> motors (joint, gear, ctrlrange, ctrllimited).
+  Jointless bodies are fused into their parents (mjcf.py:76-98).  The
+  reference compiles the XML with MuJoCo first; this reads the same subset
+  directly.  Any other element or attribute that changes the physics raises
+  NotImplementedError; names, visuals, cameras, lights, sites and assets
+  are ignored.
+  """
+  import xml.etree.ElementTree as ET
+  root = ET.fromstring(xml)
+  if root.tag != "mujoco":
+    raise ValueError("not an MJCF document")
+  comp = root.find("compiler")
+  comp = comp.attrib if comp is not None else {}
+  degrees = comp.get("angle", "degree") == "degree"
+  if comp.get("coordinate", "local") != "local":
+    raise NotImplementedError("global coordinates")
+  if comp.get("inertiafromgeom", "true") not in ("true", "auto"):
+    raise NotImplementedError("inertia other than from geoms")
+  opt = root.find("option")
+  opt = opt.attrib if opt is not None else {}
+  if set(opt) - {"timestep", "iterations", "gravity"}:
+    raise NotImplementedError(f"<option> attributes {sorted(set(opt))}")
+  custom = {}
+  for c in root.findall("custom"):
+    for num in c.findall("numeric"):
+      custom[num.get("name")] = _floats(num.get("data"))
+  defaults = {"joint": {}, "geom": {}, "motor": {}}
+  for d in root.findall("default"):
+    if d.get("class") or d.findall("default"):
+      raise NotImplementedError("named or nested default classes")
+    for k in defaults:
+      e = d.find(k)
+      if e is not None:
+        defaults[k].update(e.attrib)
+  b = ModelBuilder(
+      name or root.get("model", "mjcf"), timestep=float(opt.get("timestep", 0.002)),
+      iterations=int(opt.get("iterations", 100)),
+      solver_maxls=int(custom.get("solver_maxls", [20])[0]),
+      matrix_inv_iterations=int(custom.get("matrix_inv_iterations", [10])[0]),
+      gravity=_floats(opt.get("gravity", "0 0 -9.81")))
+  scale = np.pi / 180.0 if degrees else 1.0
+
+  def geom(body: int, e):
+    a = dict(defaults["geom"], **e.attrib)
+    _mjcf_check("geom", a)
+    typ = a.get("type", "sphere")
+    if typ not in _MJCF_GEOMS:
+      raise NotImplementedError(f"geom type {typ!r}")
+    size = _floats(a.get("size", "0"))
+    kw = dict(contype=int(a.get("contype", 1)), conaffinity=int(a.get("conaffinity", 1)),
+              density=float(a.get("density", 1000.0)), name=a.get("name", ""))
+    fr = _floats(a.get("friction", "1 0.005 0.0001"))
+    kw["friction"] = tuple(fr + [0.005, 0.0001][len(fr) - 1:]) if len(fr) < 3 else tuple(fr)
+    if "mass" in a:
+      kw["mass"] = float(a["mass"])
+    for k in ("solref", "solimp"):
+      if k in a:
+        kw[k] = tuple(_floats(a[k]))
+    if "fromto" in a:
+      kw["fromto"] = _floats(a["fromto"])
+    else:
+      kw["pos"] = _floats(a.get("pos", "0 0 0"))
+      kw["quat"] = _floats(a.get("quat", "1 0 0 0"))
+    b.add_geom(body, _MJCF_GEOMS[typ], size, **kw)
+
+  def body(e, parent: int):
+    _mjcf_check("body", e.attrib)
+    bid = b.add_body(e.get("name", ""), parent, pos=_floats(e.get("pos", "0 0 0")),
+                     quat=_floats(e.get("quat", "1 0 0 0")))
+    for c in e:
+      if c.tag == "joint":
+        a = dict(defaults["joint"], **c.attrib)
+        _mjcf_check("joint", a)
+        typ = a.get("type", "hinge")
+        if typ == "free":
+          b.add_free_joint(bid, a.get("name", "root"))
+        elif typ == "hinge":
+          rng = _floats(a["range"]) if "range" in a else None
+          limited = a.get("limited", "auto")
+          limited = (rng is not None) if limited == "auto" else limited == "true"
+          b.add_hinge(bid, a.get("name", ""), _floats(a.get("axis", "0 0 1")),
+                      pos=_floats(a.get("pos", "0 0 0")),
+                      range_rad=(rng[0] * scale, rng[1] * scale) if limited and rng else None,
+                      armature=float(a.get("armature", 0.0)),
+                      damping=float(a.get("damping", 0.0)),
+                      stiffness=float(a.get("stiffness", 0.0)),
+                      **{k: tuple(_floats(a[m])) for k, m in
+                         (("solref", "solreflimit"), ("solimp", "solimplimit")) if m in a})
+        else:
+          raise NotImplementedError(f"joint type {typ!r}")
+      elif c.tag == "geom":
+        geom(bid, c)
+      elif c.tag == "body":
+        body(c, bid)
+      elif c.tag == "inertial":
+        raise NotImplementedError("explicit inertials (inertia comes from the geoms)")
+      elif c.tag not in _MJCF_IGNORED:
+        raise NotImplementedError(f"body element <{c.tag}>")
+
+  wb = root.find("worldbody")
+  for c in wb:
+    if c.tag == "geom":
+      geom(0, c)
+    elif c.tag == "body":
+      body(c, 0)
+    elif c.tag not in _MJCF_IGNORED:
+      raise NotImplementedError(f"worldbody element <{c.tag}>")
+  for act in root.findall("actuator"):
+    for m in act:
+      if m.tag != "motor":
+        raise NotImplementedError(f"actuator <{m.tag}>")
+      a = dict(defaults["motor"], **m.attrib)
+      _mjcf_check("motor", a)
+      limited = a.get("ctrllimited", "auto")
+      rng = _floats(a["ctrlrange"]) if "ctrlrange" in a else None
+      limited = (rng is not None) if limited == "auto" else limited == "true"
+      b.add_motor(a["joint"], float(_floats(a.get("gear", "1"))[0]),
+                  tuple(rng) if limited and rng else None)
+  b.bodies = _fuse_jointless(b)
+  if "init_qpos" in custom:
+    b.init_q_override = custom["init_qpos"]
+  return b.build()
