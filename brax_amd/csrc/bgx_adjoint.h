@@ -10,7 +10,8 @@
 //
 //  A1 (one thread per env and substep of a segment): the step's own substep
 //    (substep_impl, a slot per candidate) recording its tape: tree, contact
-//    slots, accelerations and every projected-gradient iterate.
+//    slots, accelerations, every projected-gradient iterate and the
+//    products M^-1 J^T y_k, M^-1 J^T r_k its matrix applications formed.
 //  A2 (two lanes per env): the reverse sweep of the integrator, the solves
 //    with M = L^T L and the unrolled FISTA iterations, by hand, from the
 //    tape.  Output: cotangents of the position- and velocity-dependent
@@ -428,8 +429,8 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
     if constexpr (P == 1) {
       float ug[NV], ur[NV], urb[NV];
       A_apply(gb, rb, ug);  // r-bar = A g-bar
-      JT_apply(tr, ur);
-      ltl_solve<T>(t, ur);
+#pragma unroll
+      for (int d = 0; d < NV; ++d) ur[d] = rec[R::UR + k * NV + d];  // M^-1 J^T r_k
       jbar_add(gb, ur);
       jbar_add(tr, ug);
       mbar_sub(ug, ur);
@@ -438,8 +439,8 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
       if (k > 0) {  // y_0 = 0
         float uy[NV];
         load_row(tr, R::YS, k);  // y_k
-        JT_apply(tr, uy);
-        ltl_solve<T>(t, uy);
+#pragma unroll
+        for (int d = 0; d < NV; ++d) uy[d] = rec[R::UY + k * NV + d];  // M^-1 J^T y_k
         jbar_add(rb, uy);
         jbar_add(tr, urb);
         mbar_sub(urb, uy);
@@ -450,16 +451,16 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
       else for_rows([&](int r) { tr2[r] = 0.f; });  // y_0 = 0
       float ug[NV], urb[NV], uo[NV], ux[NV], c[NV];
       A_apply(gb, rb, ug);  // r-bar = A g-bar
-      // sub 0: ur = M^-1 J^T r_k;  sub 1: uy = M^-1 J^T y_k
-      JT_apply(sub ? tr2 : tr, uo);
-      ltl_solve<T>(t, uo);
+      // sub 0: ur = M^-1 J^T r_k;  sub 1: uy = M^-1 J^T y_k (both on the tape)
+#pragma unroll
+      for (int d = 0; d < NV; ++d) uo[d] = rec[(sub ? R::UY : R::UR) + k * NV + d];
 #pragma unroll
       for (int d = 0; d < NV; ++d) c[d] = sub ? ug[d] : uo[d];
       jbar_add(sub ? tr : gb, c);  // (g-bar, ur) | (r_k, ug)
       for_rows([&](int r) { Db[r] += gb[r] * tr[r]; });
       A_apply(rb, ay, urb);  // y-bar = w-bar + A r-bar
 #pragma unroll
-      for (int d = 0; d < NV; ++d) ux[d] = pair_other(uo[d]);  // sub 0 gets uy
+      for (int d = 0; d < NV; ++d) ux[d] = rec[(sub ? R::UR : R::UY) + k * NV + d];  // sub 0: uy
 #pragma unroll
       for (int d = 0; d < NV; ++d) c[d] = sub ? urb[d] : ux[d];
       jbar_add(sub ? tr2 : rb, c);  // (r-bar, uy) | (y_k, urb)

@@ -800,7 +800,8 @@ struct PrivRows {
 // optional record of a substep's intermediates (the adjoint's tape): with a
 // slot per candidate, the tape holds what the reverse sweep needs -- the
 // tree (cdof, LTL factor), the contact slots, the unconstrained and
-// constraint accelerations and every projected-gradient iterate.  The step
+// constraint accelerations, every projected-gradient iterate and the
+// products M^-1 J^T y_k, M^-1 J^T r_k of each iteration.  The step
 // kernels pass NoTape, which compiles to nothing.
 
 constexpr int ADJ_MAXITER = 8;  // most solver iterations the tape holds
@@ -837,7 +838,9 @@ struct TapeLayout {
   static constexpr int YS = BETA + ADJ_MAXITER;       // [ADJ_MAXITER][NR] y_k
   static constexpr int RS = YS + ADJ_MAXITER * NR;    // [ADJ_MAXITER][NR] A y_k + b
   static constexpr int XS = RS + ADJ_MAXITER * NR;    // [ADJ_MAXITER][NR] x_{k+1}
-  static constexpr int SIZE = XS + ADJ_MAXITER * NR;
+  static constexpr int UY = XS + ADJ_MAXITER * NR;    // [ADJ_MAXITER][NV] M^-1 J^T y_k
+  static constexpr int UR = UY + ADJ_MAXITER * NV;    // [ADJ_MAXITER][NV] M^-1 J^T r_k
+  static constexpr int SIZE = UR + ADJ_MAXITER * NV;
 };
 
 struct NoTape {
@@ -1099,10 +1102,13 @@ BGX_D bool substep_impl(const float* __restrict__ M, float* q, float* qd,
     for (int i = 0; i < T::NLIM; ++i) tau[T::LIM_D[i]] += lside[i] * in_lim(i);
   };
 
-  auto apply_a = [&](auto in_slot, auto in_lim, auto out_slot, auto out_lim) {
+  auto no_z = [](const float*) {};
+  // out = A in; z_out receives M^-1 J^T in
+  auto apply_a = [&](auto in_slot, auto in_lim, auto out_slot, auto out_lim, auto z_out) {
     float z[NV];
     apply_jt(in_slot, in_lim, z);
     ltl_solve<T>(t, z);
+    z_out(z);
     {
       V3 va[L], vv[L];
       link_velocities<T>(t, z, va, vv);
@@ -1167,11 +1173,21 @@ BGX_D bool substep_impl(const float* __restrict__ M, float* q, float* qd,
               const float r = o + lb[i];
               lv.at(3, i) = r;
               fy += 0.5f * r * r;
+            },
+            [&](const float* z) {
+              if constexpr (Tape::ON)
+#pragma unroll
+                for (int d = 0; d < NV; ++d) tape[TL::UY + it * NV + d] = z[d];
             });
     apply_a([&](int s, int k) { return rows.at(RV_R, s, k); },
             [&](int i) { return lv.at(3, i); },
             [&](int s, int k, float o) { rows.at(RV_G, s, k) = o; },
-            [&](int i, float o) { lv.at(2, i) = o; });
+            [&](int i, float o) { lv.at(2, i) = o; },
+            [&](const float* z) {
+              if constexpr (Tape::ON)
+#pragma unroll
+                for (int d = 0; d < NV; ++d) tape[TL::UR + it * NV + d] = z[d];
+            });
     if constexpr (Tape::ON) {  // y_k and r_k = A y_k + b
       const int o = it * TL::NR;
 #pragma unroll
@@ -1214,7 +1230,8 @@ BGX_D bool substep_impl(const float* __restrict__ M, float* q, float* qd,
                 const float df = fmaxf(lv.at(1, i) - e_ * lv.at(2, i), 0.f) - lv.at(1, i);
                 dg += df * lv.at(2, i);
                 dd += df * df;
-              });
+              },
+              no_z);
       trials += 1;
       cond = e_ * (fx - fy) > e_ * dg + 0.5f * dd + lseps;
     }
@@ -1232,7 +1249,8 @@ BGX_D bool substep_impl(const float* __restrict__ M, float* q, float* qd,
               const float xp = fmaxf(lv.at(1, i) - e_ * lv.at(2, i), 0.f);
               const float dv = fmaxf(xp - o, 0.f) - xp;
               e2 += dv * dv;
-            });
+            },
+            no_z);
     err = fsqrt(e2);
     const float tn = 0.5f * (1.f + fsqrt(1.f + 4.f * tk * tk));
     const float beta = (tk - 1.f) * rcp(tn);
