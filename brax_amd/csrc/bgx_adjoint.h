@@ -193,7 +193,7 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
   };
   // f(r) for every active row r
   auto for_rows = [&](auto&& f) {
-#pragma unroll 1
+#pragma unroll 2
     for (int r = 0; r < 4 * n_c; ++r) f(r);
 #pragma unroll
     for (int i = 0; i < T::NLIM; ++i) f(NRC + i);
@@ -201,7 +201,7 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
   // tape row vector of iteration k into dst (4 loads in flight per slot)
   auto load_row = [&](const SV& dst, int base, int k) {
     const int b0 = base + k * NR;
-#pragma unroll 1
+#pragma unroll 2
     for (int s = 0; s < n_c; ++s) {
       const float v0 = rec[b0 + 4 * s], v1 = rec[b0 + 4 * s + 1];
       const float v2 = rec[b0 + 4 * s + 2], v3_ = rec[b0 + 4 * s + 3];
@@ -258,7 +258,7 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
   // J z on the active rows
   auto J_apply = [&](const float* z, const SV& out) {
     store_link_vel(z);
-#pragma unroll 1
+#pragma unroll 2
     for (int s = 0; s < n_c; ++s) {
       V3 a, v;
       slot_vel(s, a, v);
@@ -282,7 +282,7 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
   };
   auto JT_apply = [&](const SV& x, float* tau) {
     clear_link_force();
-#pragma unroll 1
+#pragma unroll 2
     for (int s = 0; s < n_c; ++s) {
       float a0, a1, a2;
       const V3 w = slot_force(x, s, a0, a1, a2);
@@ -354,7 +354,7 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
   auto jbar_add = [&](const SV& a, const float* c) {
     store_link_vel(c);
     clear_link_force();
-#pragma unroll 1
+#pragma unroll 2
     for (int s = 0; s < n_c; ++s) {
       float a0, a1, a2;
       const V3 f = slot_force(a, s, a0, a1, a2);
