@@ -1110,8 +1110,10 @@ _MJCF_ATTRS = {
     "joint": {"type", "axis", "pos", "range", "limited", "armature", "damping",
               "stiffness", "solreflimit", "solimplimit", "margin"},
     "body": {"pos", "quat", "childclass"},
-    "motor": {"joint", "gear", "ctrlrange", "ctrllimited"},
+    "motor": {"joint", "gear", "ctrlrange", "ctrllimited", "forcerange", "forcelimited"},
 }
+_MJCF_ATTRS["position"] = _MJCF_ATTRS["motor"] | {"kp", "kv"}
+_MJCF_ATTRS["velocity"] = _MJCF_ATTRS["motor"] | {"kv"}
 
 
 def _mjcf_check(tag: str, a: dict) -> None:
@@ -1141,7 +1143,8 @@ def load_mjcf(xml: str, name: Optional[str] = None) -> System:
   joints (axis, pos, range, limited, armature, damping, stiffness,
   solreflimit, solimplimit), plane / sphere / capsule geoms (size, pos,
   quat, fromto, density, mass, contype, conaffinity, friction, solref,
-  solimp) and <actuator> motors (joint, gear, ctrlrange, ctrllimited).
+  solimp) and <actuator> motor / position / velocity actuators (joint,
+  gear, ctrlrange, ctrllimited, forcerange, forcelimited, kp, kv).
   Jointless bodies are fused into their parents (mjcf.py:76-98).  The
   reference compiles the XML with MuJoCo first; this reads the same subset
   directly.  Any other element or attribute that changes the physics raises
@@ -1159,6 +1162,11 @@ def load_mjcf(xml: str, name: Optional[str] = None) -> System:
     raise NotImplementedError("global coordinates")
   if comp.get("inertiafromgeom", "true") not in ("true", "auto"):
     raise NotImplementedError("inertia other than from geoms")
+  autolimits = comp.get("autolimits", "true") == "true"
+
+  def limited(a: dict, flag: str, rng) -> bool:
+    v = a.get(flag, "auto")
+    return (rng is not None and autolimits) if v == "auto" else v == "true"
   opt = root.find("option")
   opt = opt.attrib if opt is not None else {}
   if set(opt) - {"timestep", "iterations", "gravity"}:
@@ -1167,7 +1175,7 @@ def load_mjcf(xml: str, name: Optional[str] = None) -> System:
   for c in root.findall("custom"):
     for num in c.findall("numeric"):
       custom[num.get("name")] = _floats(num.get("data"))
-  defaults = {"joint": {}, "geom": {}, "motor": {}}
+  defaults = {"joint": {}, "geom": {}, "motor": {}, "position": {}, "velocity": {}}
   for d in root.findall("default"):
     if d.get("class") or d.findall("default"):
       raise NotImplementedError("named or nested default classes")
@@ -1219,11 +1227,10 @@ def load_mjcf(xml: str, name: Optional[str] = None) -> System:
           b.add_free_joint(bid, a.get("name", "root"))
         elif typ == "hinge":
           rng = _floats(a["range"]) if "range" in a else None
-          limited = a.get("limited", "auto")
-          limited = (rng is not None) if limited == "auto" else limited == "true"
+          lim = limited(a, "limited", rng)
           b.add_hinge(bid, a.get("name", ""), _floats(a.get("axis", "0 0 1")),
                       pos=_floats(a.get("pos", "0 0 0")),
-                      range_rad=(rng[0] * scale, rng[1] * scale) if limited and rng else None,
+                      range_rad=(rng[0] * scale, rng[1] * scale) if lim and rng else None,
                       armature=float(a.get("armature", 0.0)),
                       damping=float(a.get("damping", 0.0)),
                       stiffness=float(a.get("stiffness", 0.0)),
@@ -1248,17 +1255,27 @@ def load_mjcf(xml: str, name: Optional[str] = None) -> System:
       body(c, 0)
     elif c.tag not in _MJCF_IGNORED:
       raise NotImplementedError(f"worldbody element <{c.tag}>")
+  # actuators (mjcf.py:394-423): motor gain 1; position kp: gain kp, bias
+  # -kp q (and -kv qd); velocity kv: gain kv, bias -kv qd
   for act in root.findall("actuator"):
     for m in act:
-      if m.tag != "motor":
+      if m.tag not in ("motor", "position", "velocity"):
         raise NotImplementedError(f"actuator <{m.tag}>")
-      a = dict(defaults["motor"], **m.attrib)
-      _mjcf_check("motor", a)
-      limited = a.get("ctrllimited", "auto")
-      rng = _floats(a["ctrlrange"]) if "ctrlrange" in a else None
-      limited = (rng is not None) if limited == "auto" else limited == "true"
-      b.add_motor(a["joint"], float(_floats(a.get("gear", "1"))[0]),
-                  tuple(rng) if limited and rng else None)
+      a = dict(defaults[m.tag], **m.attrib)
+      _mjcf_check(m.tag, a)
+      crng = _floats(a["ctrlrange"]) if "ctrlrange" in a else None
+      frng = _floats(a["forcerange"]) if "forcerange" in a else None
+      gain, bq, bqd = 1.0, 0.0, 0.0
+      if m.tag == "position":
+        gain = float(a.get("kp", 1.0))
+        bq, bqd = -gain, -float(a.get("kv", 0.0))
+      elif m.tag == "velocity":
+        gain = float(a.get("kv", 1.0))
+        bqd = -gain
+      b.add_actuator(a["joint"], float(_floats(a.get("gear", "1"))[0]), gain=gain,
+                     bias_q=bq, bias_qd=bqd,
+                     ctrlrange=tuple(crng) if limited(a, "ctrllimited", crng) and crng else None,
+                     forcerange=tuple(frng) if limited(a, "forcelimited", frng) and frng else None)
   b.bodies = _fuse_jointless(b)
   if "init_qpos" in custom:
     b.init_q_override = custom["init_qpos"]

@@ -89,7 +89,7 @@ def test_radian_angles_and_joint_defaults():
     ('type="hinge"', 'type="slide"'),
     ('type="sphere"', 'type="box"'),
     ('<joint name="hinge3"', '<inertial mass="1" pos="0 0 0"/><joint name="hinge3"'),
-    ('<motor joint', '<position joint'),
+    ('<motor joint', '<general joint'),
 ])
 def test_outside_subset_raises(edit):
   with pytest.raises(NotImplementedError):
@@ -162,3 +162,25 @@ def test_reference_humanoid_names():
   assert s.q_size() == 24 and s.qd_size() == 23 and s.act_size() == 17
   assert s.opt.timestep == pytest.approx(0.003) and s.solver_iterations == 8
   assert s.matrix_inv_iterations == 20 and s.solver_maxls == 15
+
+
+@_needs_ref
+def test_reference_actuator_fixtures():
+  """Position / velocity / motor actuators of brax/test_data (as
+  actuator_test.py:50-123 loads them) compile to the builder's servos:
+  position kp -> gain kp, bias_q -kp; velocity kv -> gain kv, bias_qd -kv
+  (brax/io/mjcf.py:394-423 reading MuJoCo's gainprm / biasprm)."""
+  a = _load("test_data/single_pendulum_position_frclimit.xml")
+  b = S.pendulum(1, position_servo=(10.0, 10.0, (-2.5, 3.1)), timestep=0.001)
+  _same_physics(a, b)
+  a = _load("test_data/single_pendulum_motor.xml")
+  b = S.pendulum(1, motor_gear=150.0, ctrlrange=(-100.0, 100.0), timestep=0.001)
+  _same_physics(a, b)
+  v = _load("test_data/single_pendulum_velocity.xml").actuator
+  np.testing.assert_allclose([v.gain[0], v.bias_q[0], v.bias_qd[0]], [10.0, 0.0, -10.0])
+  assert np.isinf(v.ctrl_range).all() and np.isinf(v.force_range).all()
+  p = _load("test_data/single_pendulum_position.xml").actuator
+  np.testing.assert_allclose([p.gain[0], p.gear[0], p.bias_q[0]], [10.0, 1.0, -10.0])
+  t = _load("test_data/triple_pendulum_motor.xml")
+  assert t.act_size() == 3 and t.link_types == "111"
+  np.testing.assert_allclose(t.actuator.ctrl_range, np.tile([-100.0, 100.0], (3, 1)))
