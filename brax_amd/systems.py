@@ -1101,6 +1101,10 @@ def from_arrays_check(sys: System) -> None:
 
 _MJCF_GEOMS = {"plane": GEOM_PLANE, "sphere": GEOM_SPHERE, "capsule": GEOM_CAPSULE}
 _MJCF_IGNORED = {"camera", "light", "site"}
+# top-level elements read (compiler ... actuator) or without effect on the
+# step (assets, sizes, visuals, statistics, sensors, keyframes)
+_MJCF_TOP = {"compiler", "option", "custom", "default", "worldbody", "actuator", "asset",
+             "size", "visual", "statistic", "sensor", "keyframe"}
 # attributes with no effect on the generalized step: names, visuals, and
 # fields the step does not read (a free joint's margin, geom user data)
 _MJCF_VISUAL = {"name", "rgba", "material", "user", "group"}
@@ -1155,6 +1159,9 @@ def load_mjcf(xml: str, name: Optional[str] = None) -> System:
   root = ET.fromstring(xml)
   if root.tag != "mujoco":
     raise ValueError("not an MJCF document")
+  for c in root:
+    if c.tag not in _MJCF_TOP:
+      raise NotImplementedError(f"top-level element <{c.tag}>")
   comp = root.find("compiler")
   comp = comp.attrib if comp is not None else {}
   degrees = comp.get("angle", "degree") == "degree"
@@ -1168,6 +1175,8 @@ def load_mjcf(xml: str, name: Optional[str] = None) -> System:
     v = a.get(flag, "auto")
     return (rng is not None and autolimits) if v == "auto" else v == "true"
   opt = root.find("option")
+  if opt is not None and len(opt):
+    raise NotImplementedError("<option> children (flags, overrides)")
   opt = opt.attrib if opt is not None else {}
   if set(opt) - {"timestep", "iterations", "gravity"}:
     raise NotImplementedError(f"<option> attributes {sorted(set(opt))}")
@@ -1247,14 +1256,14 @@ def load_mjcf(xml: str, name: Optional[str] = None) -> System:
       elif c.tag not in _MJCF_IGNORED:
         raise NotImplementedError(f"body element <{c.tag}>")
 
-  wb = root.find("worldbody")
-  for c in wb:
-    if c.tag == "geom":
-      geom(0, c)
-    elif c.tag == "body":
-      body(c, 0)
-    elif c.tag not in _MJCF_IGNORED:
-      raise NotImplementedError(f"worldbody element <{c.tag}>")
+  for wb in root.findall("worldbody"):
+    for c in wb:
+      if c.tag == "geom":
+        geom(0, c)
+      elif c.tag == "body":
+        body(c, 0)
+      elif c.tag not in _MJCF_IGNORED:
+        raise NotImplementedError(f"worldbody element <{c.tag}>")
   # actuators (mjcf.py:394-423): motor gain 1; position kp: gain kp, bias
   # -kp q (and -kv qd); velocity kv: gain kv, bias -kv qd
   for act in root.findall("actuator"):

@@ -107,6 +107,9 @@ def test_not_mjcf_raises():
     ('contype="0" conaffinity="0"/>', 'contype="0" conaffinity="0" condim="1"/>'),
     ('<joint name="hinge2"', '<joint name="hinge2" margin="0.1"'),
     ('<mujoco model="triple pendulum">', '<mujoco><compiler coordinate="global"/>'),
+    ('<actuator>', '<equality><joint joint1="hinge1" joint2="hinge2"/></equality><actuator>'),
+    ('<actuator>', '<include file="more.xml"/><actuator>'),
+    ('<option timestep="0.001"/>', '<option timestep="0.001"><flag contact="disable"/></option>'),
 ])
 def test_unread_physics_attributes_raise(edit):
   with pytest.raises(NotImplementedError):
