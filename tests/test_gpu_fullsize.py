@@ -1,0 +1,77 @@
+"""The step at its full batch sizes: 65536 quadrupeds and 16384 humanoids.
+
+Size-independent properties instead of a full oracle run: each env's result
+does not depend on the batch around it (a seeded sample of 200 envs
+stepped as a batch of their own gives bitwise the same states), and the
+sample agrees with the float64 oracle within |gpu - ref| <= 1e-5 +
+1e-4 |ref|.  Quadruped: every env, except envs where the float32 oracle
+itself leaves that band (a line-search or contact decision that float32
+rounding flips), at most one in eight.  Humanoid: a float32 run of the
+oracle leaves the band on about one env in ten of this input distribution
+(stacked 3-dof hips and 13 contacts make its solver decisions sensitive to
+rounding), so the GPU may leave it on no more envs than the float32 oracle
+does, plus one in sixteen; the median env is within half the band.
+"""
+import numpy as np
+import pytest
+import torch
+
+from brax_amd import systems as S
+from brax_amd.generalized import pipeline
+from oracle.cstep import CStep
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda", 0)
+RTOL, ATOL = 1e-4, 1e-5
+
+
+def _inputs(sys, n, steps, act_scale, seed):
+  rng = np.random.default_rng(seed)
+  q0 = (sys.init_q + rng.uniform(-0.1, 0.1, (n, sys.q_size()))).astype(np.float32)
+  qd0 = (0.1 * rng.standard_normal((n, sys.qd_size()))).astype(np.float32)
+  acts = rng.uniform(-act_scale, act_scale, (steps, n, sys.act_size())).astype(np.float32)
+  return q0, qd0, acts
+
+
+def _run(sys, q0, qd0, acts):
+  t = lambda x: torch.tensor(x, device=DEV)
+  st = pipeline.init(sys, t(q0), t(qd0))
+  st, _ = pipeline.rollout(sys, st, t(acts), 5)
+  torch.cuda.synchronize()
+  return st.q.cpu().numpy(), st.qd.cpu().numpy()
+
+
+def _ratio(x, ref):
+  return (np.abs(np.asarray(x, np.float64) - ref) / (ATOL + RTOL * np.abs(ref))).max(1)
+
+
+@pytest.mark.parametrize("name,n,act_scale,strict", [("quadruped", 65536, 1.0, True),
+                                                     ("humanoid", 16384, 0.4, False)])
+def test_full_batch(name, n, act_scale, strict):
+  sys = getattr(S, name)(0)
+  steps = 1
+  q0, qd0, acts = _inputs(sys, n, steps, act_scale, seed=11)
+  q, qd = _run(sys, q0, qd0, acts)
+  assert np.isfinite(q).all() and np.isfinite(qd).all()
+  idx = np.sort(np.random.default_rng(5).choice(n, 200, replace=False))
+  qs, qds = _run(sys, q0[idx], qd0[idx], acts[:, idx])
+  np.testing.assert_array_equal(qs, q[idx])
+  np.testing.assert_array_equal(qds, qd[idx])
+  # the sample against the oracle
+  c = CStep(sys)
+  q64, qd64 = c.rollout(q0[idx].astype(np.float64), qd0[idx].astype(np.float64),
+                        acts[:, idx].astype(np.float64), 5, dtype=np.float64)
+  q32, qd32 = c.rollout(q0[idx], qd0[idx], acts[:, idx], 5, dtype=np.float32)
+  r = np.maximum(_ratio(q[idx], q64), _ratio(qd[idx], qd64))
+  r32 = np.maximum(_ratio(q32, q64), _ratio(qd32, qd64))
+  print(f"{name} n={n}: median ratio {np.median(r):.3g}; outside the band: "
+        f"gpu {(r > 1).sum()}, float32 oracle {(r32 > 1).sum()} of {len(idx)}")
+  assert np.median(r) <= 0.5
+  if strict:
+    exempt = r32 > 1
+    assert exempt.sum() <= len(idx) // 8
+    assert r[~exempt].max() <= 1.0
+  else:
+    # no more envs outside the band than a float32 run of the same
+    # algorithm has, up to a margin of one in sixteen
+    assert (r > 1).sum() <= (r32 > 1).sum() + len(idx) // 16
