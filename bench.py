@@ -416,7 +416,8 @@ def hetero_bench(args):
       "dtype": "fp32",
       "data": "synthetic (seeded chain and quadruped states, U(-1,1) actions in HBM)",
       "config": {"workload": "heterogeneous batch: one HIP graph of per-topology "
-                             "rollout kernels on 4 streams", "envs": n,
+                             f"rollout kernels on {len(hb.streams())} streams",
+                 "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "4"), "envs": n,
                  "env_steps": K, "groups": len(hb.groups),
                  "envs_per_workgroup": None if epw is None else
                  [[envs[g.envs[0]].name, g.n, e] for g, e in zip(hb.groups, epw)],

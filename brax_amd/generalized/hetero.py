@@ -6,14 +6,25 @@ The reference vmaps one System over a batch (brax/envs/wrappers/training.py:
 of envs that share a compiled topology.  A group of one shared System uses a
 single model table; a group of per-env systems (``SystemBatch``, e.g. chains
 with their own lengths and masses) uses one table per env.  ``rollout``
-launches every group's kernel on its own stream so the groups run
-concurrently; ``graph()`` captures all of them in one HIP graph, replayed as
+launches the groups' kernels on several streams so the groups run
+concurrently; a capture of ``launch_rollout`` is one HIP graph, replayed as
 one launch.
+
+A group's kernel takes about as long as one of its waves (a lane per env,
+the waves run side by side), so the batch takes as long as its busiest
+stream.  There is one stream per hardware queue of the process (the HIP
+runtime's GPU_MAX_HW_QUEUES, 4 by default; streams beyond the queues would
+share them and run in turn); the groups go to the streams longest first,
+each to the least loaded one.  More queues do not fit the long chains:
+each queue reserves scratch for the largest private segment it runs (9.7 KB
+per lane for the rolled 25-32-link kernels), and at 8 queues the runtime
+fails to allocate them.
 """
 
 from __future__ import annotations
 
 import ctypes
+import os
 from typing import Dict, List, Sequence
 
 import torch
@@ -22,6 +33,22 @@ from brax_amd import _lib
 from brax_amd import codegen
 from brax_amd import systems as S
 from brax_amd.generalized import pipeline
+
+
+# the step kernels unroll their dof loops up to this many dofs
+# (bgx_step.h unroll_nv); longer trees run rolled loops through scratch
+_UNROLL_MAX_DOFS = 24
+
+
+def _group_cost(g) -> float:
+  """Relative time of one wave of a group's kernel: dofs^3, four times that
+  with contact or limit rows or with rolled dof loops (kernel traces: a
+  14-dof quadruped 11 ms, a 24-link chain 16 ms, a 25-link chain 60 ms per
+  20 env steps)."""
+  s0 = g.sys.systems[0] if hasattr(g.sys, "systems") else g.sys
+  nv = s0.qd_size()
+  rows = s0.num_contacts() + s0.num_limit_rows()
+  return float(nv ** 3) * (4.0 if rows > 0 or nv > _UNROLL_MAX_DOFS else 1.0)
 
 
 class _Group:
@@ -78,9 +105,20 @@ class HeteroBatch:
     """Row of env from per-group tensors."""
     return per_group[int(self.env_group[env])][int(self.env_row[env])]
 
-  def streams(self, k: int = 4):
+  def streams(self, k: int = 0):
+    """The batch's streams (k of them; 0: one per hardware queue, at most one
+    per group) and each group's stream."""
     if self._streams is None:
+      if k <= 0:
+        k = int(os.environ.get("GPU_MAX_HW_QUEUES", "") or 4)
+      k = max(1, min(k, len(self.groups)))
       self._streams = [torch.cuda.Stream(device=self.device) for _ in range(k)]
+      load = [0.0] * k
+      self._stream_of = [0] * len(self.groups)
+      for g in sorted(range(len(self.groups)), key=lambda g: -_group_cost(self.groups[g])):
+        j = min(range(k), key=lambda j: load[j])
+        self._stream_of[g] = j
+        load[j] += _group_cost(self.groups[g])
     return self._streams
 
   def launch_rollout(self, q: List[torch.Tensor], qd: List[torch.Tensor],
@@ -94,7 +132,7 @@ class HeteroBatch:
     for s in streams:
       s.wait_stream(cur)
     for g, grp in enumerate(self.groups):
-      s = streams[g % len(streams)]
+      s = streams[self._stream_of[g]]
       m = grp.model
       a = acts[g] if m.nu else None
       _lib.check(lib.bgx_rollout(
