@@ -51,6 +51,18 @@ def _group_cost(g) -> float:
   return float(nv ** 3) * (4.0 if rows > 0 or nv > _UNROLL_MAX_DOFS else 1.0)
 
 
+def assign_streams(costs: Sequence[float], k: int) -> List[int]:
+  """Stream of each job: longest first, each to the least loaded of k
+  streams (ties to the lowest stream)."""
+  load = [0.0] * k
+  out = [0] * len(costs)
+  for g in sorted(range(len(costs)), key=lambda g: -costs[g]):
+    j = min(range(k), key=lambda j: load[j])
+    out[g] = j
+    load[j] += costs[g]
+  return out
+
+
 class _Group:
 
   def __init__(self, envs: List[int], sys_obj, device):
@@ -113,12 +125,7 @@ class HeteroBatch:
         k = int(os.environ.get("GPU_MAX_HW_QUEUES", "") or 4)
       k = max(1, min(k, len(self.groups)))
       self._streams = [torch.cuda.Stream(device=self.device) for _ in range(k)]
-      load = [0.0] * k
-      self._stream_of = [0] * len(self.groups)
-      for g in sorted(range(len(self.groups)), key=lambda g: -_group_cost(self.groups[g])):
-        j = min(range(k), key=lambda j: load[j])
-        self._stream_of[g] = j
-        load[j] += _group_cost(self.groups[g])
+      self._stream_of = assign_streams([_group_cost(g) for g in self.groups], k)
     return self._streams
 
   def launch_rollout(self, q: List[torch.Tensor], qd: List[torch.Tensor],
