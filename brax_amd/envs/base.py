@@ -61,7 +61,7 @@ class PipelineEnv(Env):
   def __init__(self, sys: S.System, backend: str = "generalized", n_frames: int = 1,
                device: Optional[torch.device] = None, debug: bool = False):
     if backend != "generalized":
-      raise ValueError(f"backend {backend!r} is not available: this build runs the "
+      raise ValueError(f"backend {backend!r} is not available: brax_amd runs the "
                        "generalized pipeline")
     self.sys = sys
     self.backend = backend

@@ -32,7 +32,7 @@ enum {
   BGX_ERR_ARG = 1,      /* null pointer, bad size or count */
   BGX_ERR_TOPOLOGY = 2, /* topology id not compiled into this library */
   BGX_ERR_HIP = 3,      /* HIP runtime error (allocation, launch) */
-  BGX_ERR_DISABLED = 4, /* entry point switched off in this build */
+  BGX_ERR_DISABLED = 4, /* entry point switched off at compile time */
 };
 
 /* per-env status bits written by bgx_step / bgx_rollout */
