@@ -60,15 +60,24 @@ def topology_structs(inc_path: str):
 CACHE = os.path.join(HERE, "_build")
 
 
-def _inputs_digest() -> str:
+def _inputs_digest(unit_src: str) -> str:
+  """Digest of what one object file is built from: its own .hip file, the
+  shared headers under csrc/ (every .h and the generated topologies.inc),
+  the public header and the flags.  A change to one .hip file rebuilds only
+  its objects."""
   h = hashlib.sha1()
   for name in sorted(os.listdir(CSRC)):
+    if name.endswith(".hip"):
+      continue
     h.update(name.encode())
     with open(os.path.join(CSRC, name), "rb") as f:
       h.update(f.read())
   with open(os.path.join(os.path.dirname(HERE), "include", "brax_gen.h"), "rb") as f:
     h.update(f.read())
   h.update(" ".join(HIPCC_FLAGS).encode())
+  h.update(os.path.basename(unit_src).encode())
+  with open(unit_src, "rb") as f:
+    h.update(f.read())
   return h.hexdigest()[:16]
 
 
@@ -81,14 +90,13 @@ def build(verbose: bool = False, resource_usage: bool = False, jobs: int = 0,
   structs = topology_structs(inc)
   extra = ["-Rpass-analysis=kernel-resource-usage"] if resource_usage else []
   jobs = jobs or min(8, (os.cpu_count() or 4), len(structs) + 1)
-  digest = _inputs_digest()
   os.makedirs(CACHE, exist_ok=True)
   units = [(os.path.join(CSRC, "bgx.hip"), "bgx", [])]
   for s in structs:
     units.append((os.path.join(CSRC, "bgx_topo.hip"), s, [f"-DBGX_TOPO={s}"]))
   objs, cmds = [], []
   for src, name, d in units:
-    obj = os.path.join(CACHE, f"{name}-{digest}.o")
+    obj = os.path.join(CACHE, f"{name}-{_inputs_digest(src)}.o")
     objs.append(obj)
     if cached and os.path.exists(obj) and not resource_usage:
       continue
@@ -104,8 +112,9 @@ def build(verbose: bool = False, resource_usage: bool = False, jobs: int = 0,
       print(log, flush=True)
   os.replace(out, LIB)
   # drop objects of other input versions
+  keep = {os.path.basename(o) for o in objs}
   for f in os.listdir(CACHE):
-    if f.endswith(".o") and f"-{digest}." not in f:
+    if f.endswith(".o") and f not in keep:
       os.remove(os.path.join(CACHE, f))
   return LIB
 

@@ -106,7 +106,11 @@ struct LVec {
 };
 
 // the value of the other lane of an env's lane pair
+#ifdef BGX_HOST_CHECK
+BGX_D float pair_other(float v) { return v; }  // the host build runs one lane per env
+#else
 BGX_D float pair_other(float v) { return __shfl_xor(v, 1); }
+#endif
 
 BGX_D V3 ld3(const RecPtr& rec, int f) { return v3(rec[f], rec[f + 1], rec[f + 2]); }
 
@@ -131,7 +135,8 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
   using Y = RevLds<T>;
   constexpr int NV = T::NV, L = T::L, NR = R::NR, NRC = R::NRC;
   static_assert(P == 1 || P == 2, "one or two lanes per env");
-  const int n_c = (int)rec[R::NC], iters = (int)rec[R::ITERS];
+  const int n_c = tape_int(rec[R::NC], 0, R::KS);
+  const int iters = tape_int(rec[R::ITERS], 0, ADJ_MAXITER);
   // products both lanes form alike are counted by lane 0 only
   const float own = (P == 1 || sub == 0) ? 1.f : 0.f;
   constexpr int ES = AL / P;  // envs per workgroup: stride of the shared region
@@ -250,7 +255,7 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
     }
   };
   auto slot_vel = [&](int s, V3& w, V3& v) {
-    const float* p = &lv[6 * (int)sl[14 * s]];
+    const float* p = &lv[6 * tape_int(sl[14 * s], 0, L - 1)];
     w = v3(p[0], p[AL], p[2 * AL]);
     v = v3(p[3 * AL], p[4 * AL], p[5 * AL]);
   };
@@ -286,7 +291,7 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
     for (int s = 0; s < n_c; ++s) {
       float a0, a1, a2;
       const V3 w = slot_force(x, s, a0, a1, a2);
-      add_link_force((int)sl[14 * s], cross(slot_v3(s, 1), w), w);
+      add_link_force(tape_int(sl[14 * s], 0, L - 1), cross(slot_v3(s, 1), w), w);
     }
     V3 fa[L], fv[L];
     load_link_force(fa, fv);
@@ -368,7 +373,7 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
       g[3 * AL] += a1 * vp.x, g[4 * AL] += a1 * vp.y, g[5 * AL] += a1 * vp.z;
       g[6 * AL] += a2 * vp.x, g[7 * AL] += a2 * vp.y, g[8 * AL] += a2 * vp.z;
       g[9 * AL] += rf.x, g[10 * AL] += rf.y, g[11 * AL] += rf.z;
-      add_link_force((int)sl[14 * s], cross(r, f), f);
+      add_link_force(tape_int(sl[14 * s], 0, L - 1), cross(r, f), f);
     }
     V3 fa[L], fv[L];
     load_link_force(fa, fv);
@@ -590,14 +595,14 @@ BGX_D float adjoint_b_q(const float* __restrict__ M, const float* q_in, const fl
   // contact rows of the active candidates
 #pragma unroll
   for (int c = 0; c < T::NCAND; ++c) {
-    const int s = (int)rec[R::CSLOT + c];
+    const int s = tape_int(rec[R::CSLOT + c], -1, R::KS - 1);
     if (s >= 0) {
       const int l = T::CAND_LINK[c];
       const float* cd = cand_rec<T>(M, c);
       V3T<S> ctr, axis, pos, t1, t2;
       candidate_center<T>(M, c, xp, xr, ctr, axis);
       const S dist = candidate_dist<T>(M, c, ctr);
-      contact_frame<T>(M, c, ctr, axis, dist, (int)rec[R::CFB + c], pos, t1, t2);
+      contact_frame<T>(M, c, ctr, axis, dist, tape_int(rec[R::CFB + c], 0, 1), pos, t1, t2);
       const V3T<S> n = lift3<S>(PZ ? v3(0.f, 0.f, 1.f) : ldv3(cd + CD_PN));
       const float mu = cd[CD_MU];
       const V3T<S> r = pos - rc;
@@ -723,14 +728,14 @@ BGX_D float adjoint_b_qd(const float* __restrict__ M, const float* q_in, const f
   // contact rows: aref = -b (J qd) - k imp dist, its velocity term
 #pragma unroll
   for (int c = 0; c < T::NCAND; ++c) {
-    const int s = (int)rec[R::CSLOT + c];
+    const int s = tape_int(rec[R::CSLOT + c], -1, R::KS - 1);
     if (s >= 0) {
       const int l = T::CAND_LINK[c];
       const float* cd = cand_rec<T>(M, c);
       V3 ctr, axis, pos, t1, t2;
       candidate_center<T>(M, c, xp, xr, ctr, axis);
       const float dist = candidate_dist<T>(M, c, ctr);
-      contact_frame<T>(M, c, ctr, axis, dist, (int)rec[R::CFB + c], pos, t1, t2);
+      contact_frame<T>(M, c, ctr, axis, dist, tape_int(rec[R::CFB + c], 0, 1), pos, t1, t2);
       const V3 n = PZ ? v3(0.f, 0.f, 1.f) : ldv3(cd + CD_PN);
       const float mu = cd[CD_MU];
       const V3T<S> r = lift3<S>(pos - rc);

@@ -7,18 +7,31 @@
 // Dual (value + directional derivative).
 #pragma once
 
+#ifdef BGX_HOST_CHECK
+// A host build of the same device code for memory-safety tests on the CPU
+// (tests/host/adj_tape_check.cpp, compiled by g++ with address and
+// undefined-behaviour sanitizers); the library never defines this.
+#include <cmath>
+#define BGX_D inline
+#else
 #include <hip/hip_runtime.h>
-
 #define BGX_D __device__ __forceinline__
+#endif
 
 namespace bgx {
 
 // hardware reciprocal / square root / reciprocal square root (1 ulp); the
 // step's divisions are all normalizations or scale factors whose rounding is
 // far below the parity tolerance
+#ifdef BGX_HOST_CHECK
+BGX_D float rcp(float x) { return 1.f / x; }
+BGX_D float fsqrt(float x) { return std::sqrt(x); }
+BGX_D float rsq(float x) { return 1.f / std::sqrt(x); }
+#else
 BGX_D float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 BGX_D float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 BGX_D float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+#endif
 
 // ---------------------------------------------------------------------------
 // forward-mode dual number: v + d eps, eps^2 = 0

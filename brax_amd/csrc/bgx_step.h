@@ -855,6 +855,14 @@ struct RecPtr {
   BGX_D float& operator[](int f) const { return p[(long long)f * n]; }
 };
 
+// an integer field of a tape (a count, slot or link index): the float
+// rounded toward zero and clamped to [lo, hi], NaN to lo.  The forward
+// writes in-range values; the clamp keeps a damaged tape from indexing
+// outside the arrays its readers address.
+BGX_D int tape_int(float v, int lo, int hi) {
+  return v >= (float)lo ? (v <= (float)hi ? (int)v : hi) : lo;
+}
+
 // pyramid directions of a slot in the generic form (n, mu t1, mu t2)
 BGX_D void slot_dirs(const SlotGeo<true>& g, V3& d0, V3& d1, V3& d2) {
   d0 = v3(0.f, 0.f, 1.f), d1 = v3(g.ax, g.ay, 0.f), d2 = v3(-g.ay, g.ax, 0.f);

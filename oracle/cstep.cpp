@@ -45,6 +45,10 @@ struct Sys {
   // plane body link, plane pos[3] quat[4], geom link, pos[3], quat[4],
   // radius, half length, mu, solref[2], solimp[5], link_a, link_b
   const double* con;      // [nc][CON]
+  // capsule tangent frame: 0 = the axis projected on the plane (the
+  // restatement's reading of MJX), 1 = orthogonals(n) as for a sphere (an
+  // alternative reading, for sensitivity studies of this unpinned detail)
+  int capsule_frame;
 };
 enum { C_KIND, C_SIGN, C_PL, C_PP, C_PQ = C_PP + 3, C_GL = C_PQ + 4, C_GP, C_GQ = C_GP + 3,
        C_R = C_GQ + 4, C_HL, C_MU, C_SOLREF, C_SOLIMP = C_SOLREF + 2, C_LA = C_SOLIMP + 5,
@@ -460,6 +464,11 @@ template <class R> int jacobian(const Sys& s, const R* q, const R* qd, Work<R>& 
       R bn;
       V3<R> bb = normalize3(axis - n * dot(n, axis), &bn);
       if (bn < R(0.5)) bb = (R(-0.5) < n.y && n.y < R(0.5)) ? V3<R>{0, 1, 0} : V3<R>{0, 0, 1};
+      if (s.capsule_frame == 1) {  // alternative: orthogonals(normalize(n))
+        const V3<R> a = normalize3(n);
+        V3<R> b = (R(-0.5) < a.y && a.y < R(0.5)) ? V3<R>{0, 1, 0} : V3<R>{0, 0, 1};
+        bb = normalize3(b - a * dot(a, b));
+      }
       f1 = bb, f2 = cross(n, bb);
       ctr = gpos + axis * ((R)cr[C_HL] * (R)cr[C_SIGN]);
     }

@@ -27,7 +27,8 @@ class _Sys(ctypes.Structure):
                                 "solver_iters", "solver_maxls")] + [
       ("dt", ctypes.c_double), ("ls_eps", ctypes.c_double), ("gravity", ctypes.c_double * 3)] + [
       (n, _P) for n in ("ltype", "parent", "qadr", "dadr", "lt", "lj", "li", "inertia", "mass",
-                        "l_invw", "motion", "dofp", "solp", "act_q", "act_d", "actp", "con")]
+                        "l_invw", "motion", "dofp", "solp", "act_q", "act_d", "actp", "con")] + [
+      ("capsule_frame", _I)]
 
 
 def build(force: bool = False) -> str:
@@ -63,7 +64,11 @@ def load():
 class CStep:
   """The step of one System in C++ (float32 or float64), OpenMP over envs."""
 
-  def __init__(self, sys, ls_eps: float = float(np.finfo(np.float32).eps)):
+  def __init__(self, sys, ls_eps: float = float(np.finfo(np.float32).eps),
+               capsule_frame: str = "axis"):
+    """ls_eps: the line-search slack; capsule_frame: "axis" (the tangent
+    along the capsule axis projected on the plane) or "orthogonals" (the
+    sphere's frame), the two readings of MJX's plane-capsule frame."""
     lib = load()
     s = sys
     L, nv = s.num_links(), s.qd_size()
@@ -123,6 +128,7 @@ class CStep:
     st.mat_inv_iters = int(s.matrix_inv_iterations)
     st.solver_iters, st.solver_maxls = int(s.solver_iterations), int(s.solver_maxls)
     st.dt, st.ls_eps = float(s.opt.timestep), float(ls_eps)
+    st.capsule_frame = {"axis": 0, "orthogonals": 1}[capsule_frame]
     for k in range(3):
       st.gravity[k] = float(s.gravity[k])
     for name, arr in keep.items():

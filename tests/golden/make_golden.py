@@ -83,6 +83,32 @@ def quadruped(path):
       q1_f32=st32.q.double().numpy(), qd1_f32=st32.qd.double().numpy())
 
 
+def quadruped_ns(path):
+  """The reference's literal Newton-Schulz inverse (warm-started, with its
+  reset rule) over the 20 env steps of quadruped.npz's inputs: the state
+  after every env step, and per env the first substep whose warm start
+  reset (-1 if none)."""
+  sys = S.quadruped(0)
+  g = np.load(os.path.join(HERE, "quadruped.npz"))
+  q0, qd0, acts = g["q0"], g["qd0"], g["acts"]
+  n = q0.shape[0]
+  ns = Oracle(sys, F64, inverse="ns")
+  st = ns.init(q0, qd0)
+  eye = torch.eye(sys.qd_size(), dtype=F64)
+  first_reset = np.full(n, -1)
+  qs, qds = [], []
+  for t in range(acts.shape[0]):
+    for f in range(5):
+      x_prev = st.mass_mx_inv.clone()
+      st = ns.step(st, acts[t])
+      r = (torch.linalg.norm((eye - st.mass_mx @ x_prev).reshape(n, -1), dim=-1) > 1).numpy()
+      first_reset = np.where((first_reset < 0) & r, 5 * t + f, first_reset)
+    qs.append(st.q.numpy().copy())
+    qds.append(st.qd.numpy().copy())
+  np.savez_compressed(path, q_traj_ns=np.stack(qs), qd_traj_ns=np.stack(qds),
+                      first_reset=first_reset)
+
+
 def pendulum3(path):
   sys = S.pendulum(3)
   n, steps = 16, 20
@@ -157,6 +183,7 @@ def stages(path):
 
 if __name__ == "__main__":
   quadruped(os.path.join(HERE, "quadruped.npz"))
+  quadruped_ns(os.path.join(HERE, "quadruped_ns.npz"))
   pendulum3(os.path.join(HERE, "pendulum3.npz"))
   ant(os.path.join(HERE, "ant.npz"))
   humanoid(os.path.join(HERE, "humanoid.npz"))
