@@ -12,8 +12,13 @@ timed region replays one HIP graph holding a single rollout launch of K env
 steps (the state stays in registers between steps).
 
 Prints one JSON line: env-steps/s, the FP32 roofline of the step kernel and a
-CPU baseline (the oracle restatement in float32 on this host's cores, on a
-bounded sample of the same workload).
+CPU baseline (the oracle restatement in float32 on this host's CPU share, on
+the timed region's own inputs, all envs when they take a few seconds, else a
+leading sample); the mean active constraint-row count of the roofline's FLOP
+model comes from the same CPU run.  The line also carries ``fwd_adjoint``:
+the differentiable 128-step rollout of 4096 quadrupeds (checkpoint every 16
+env steps, the apg workload below) with its own roofline and CPU baseline
+(the torch oracle with autograd); --no-adjoint leaves it out.
 
 Other workloads (one JSON line each, same keys):
   humanoid  16384 synthetic humanoids (23 dofs, hinge triples), 1000 env
@@ -45,6 +50,9 @@ sys.path.insert(0, ROOT)
 FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 (vector) peak
 HBM_PEAK_GBS = 8000.0
 N_FRAMES = 5
+# the headline metric: forward env-steps/s at 65536 quadrupeds (value), the
+# fwd+adjoint rate (fwd_adjoint.value) and the roofline fractions
+METRIC = "env-steps/s @65536 quadrupeds; fwd+adjoint steps/s; % HBM/VALU roofline"
 
 
 def flop_per_substep(L, nv, C, n_lim, G, K, R_a):
@@ -88,38 +96,66 @@ def workload(name, n, seed=0):
 
 
 def measured_traffic(name, n, steps):
-  """HBM bytes per launch of `steps` env steps from the committed rocprofv3
-  summary of this workload (FETCH_SIZE + WRITE_SIZE, calibrated), or None."""
+  """HBM bytes of one rollout launch of `steps` env steps of `n` envs, from
+  the committed rocprofv3 --pmc summary of that same command (FETCH_SIZE +
+  WRITE_SIZE of the rollout kernel, calibrated with a copy kernel), or None
+  when no profile of this workload, batch and step count is committed."""
   path = os.path.join(ROOT, "profiles", f"k_rollout_{name}_{n}.md")
   if not os.path.exists(path):
     return None
   for line in open(path):
-    if line.startswith("traffic_bytes_per_env_step = "):
-      return float(line.split("=")[1]) * steps
+    if line.startswith("traffic_per_launch:"):
+      f = dict(kv.split("=") for kv in line.split(":", 1)[1].split())
+      if int(f["envs"]) == n and int(f["env_steps"]) == steps:
+        return float(f["bytes"])
   return None
 
 
-def cpu_baseline(sys_, q0, qd0, act_range, rng_seed=1, n=4096, steps=100):
+def host_cpu():
+  """Threads the CPU baselines use (the host share this process may run on:
+  OMP_NUM_THREADS when set, else the affinity mask) and a description of the
+  host."""
+  aff = len(os.sched_getaffinity(0))
+  threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or aff
+  model = "unknown"
+  try:
+    for line in open("/proc/cpuinfo"):
+      if line.startswith("model name"):
+        model = line.split(":", 1)[1].strip()
+        break
+  except OSError:
+    pass
+  return threads, {"nproc": os.cpu_count(), "affinity_cpus": aff, "cpu_model": model}
+
+
+def cpu_baseline(sys_, q0, qd0, acts, n_frames=N_FRAMES):
   """The oracle's C++ restatement (oracle/cstep.cpp, float32, OpenMP over
-  envs on up to 16 host threads) on the first n envs for `steps` env steps:
-  env-steps/s on this host, and the mean active constraint-row count."""
-  import os
+  envs on the host's CPU share) on the given inputs: acts (K, n_s, nu) are
+  the timed region's actions of its first n_s envs.  Returns the CPU
+  baseline object and the mean active constraint-row count per substep
+  over these inputs."""
   from oracle.cstep import CStep
-  threads = min(os.cpu_count() or 1, 16)
-  n = min(n, q0.shape[0])
+  threads, host = host_cpu()
+  K, n = acts.shape[0], acts.shape[1]
   cs = CStep(sys_)
-  rng = np.random.default_rng(rng_seed)
-  acts = rng.uniform(-act_range, act_range, (steps, n, sys_.act_size()))
-  cs.rollout(q0[:4], qd0[:4], acts[:1, :4], N_FRAMES, threads=1)  # load, warm
+  cs.rollout(q0[:4], qd0[:4], acts[:1, :4], n_frames, threads=1)  # load, warm
   t0 = time.perf_counter()
-  _, _, rows = cs.rollout(q0[:n], qd0[:n], acts, N_FRAMES, threads=threads, rows=True)
+  _, _, rows = cs.rollout(q0[:n], qd0[:n], acts, n_frames, threads=threads, rows=True)
   dt = time.perf_counter() - t0
   return {
-      "value": n * steps / dt, "unit": "env-steps/s", "cores": threads,
+      "value": n * K / dt, "unit": "env-steps/s", "cores": threads,
       "kind": "port",
-      "sample": f"{n} {sys_.name} envs x {steps} env steps ({N_FRAMES} substeps), "
-                f"oracle/cstep.cpp float32, OpenMP {threads} threads, {dt:.1f} s",
+      "sample": f"the timed region's inputs for its first {n} {sys_.name} envs x {K} env steps "
+                f"({n_frames} substeps), oracle/cstep.cpp float32, OpenMP {threads} threads, "
+                f"{dt:.1f} s",
+      "host": host,
   }, rows
+
+
+def timed_sample(n, K, budget=2_600_000):
+  """Envs of the CPU baseline's sample: all n when n x K env steps fit the
+  budget (a few seconds of the C++ oracle on 16 cores), else fewer."""
+  return max(1, min(n, budget // max(K, 1)))
 
 
 def forward_bench(args, name):
@@ -201,12 +237,18 @@ def forward_bench(args, name):
   env_steps_per_s = n * K / wall
 
   cpu = None
-  r_a = 9.9
+  r_a = None
   if not args.no_cpu:
-    cpu, r_a = cpu_baseline(sys_, q0, qd0, act_range,
-                            n=4096 if name == "quadruped" else 2048,
-                            steps=200 if name == "quadruped" else 100)
+    # the CPU baseline runs the timed region's own inputs (a leading sample
+    # of its envs when the whole batch would take too long) and gives the
+    # mean active constraint-row count over the timed steps
+    n_s = timed_sample(n, K)
+    cpu, r_a = cpu_baseline(sys_, q0[:n_s], qd0[:n_s], acts[:K, :n_s].cpu().numpy())
   nv, L = sys_.qd_size(), sys_.num_links()
+  r_a_src = "oracle/cstep.cpp float32 over the timed inputs" if r_a is not None else \
+      "not measured (--no-cpu): 9.9, the 65536-env mean of earlier runs"
+  if r_a is None:
+    r_a = 9.9
   fl = flop_per_substep(L, nv, sys_.num_contacts(), sys_.num_limit_rows(),
                         len(sys_.geom_type), sys_.solver_iterations, r_a)
   flop_launch = fl * n * K * N_FRAMES
@@ -236,7 +278,10 @@ def forward_bench(args, name):
                    "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                    "frac": achieved / FP32_PEAK_TFLOPS,
                    "traffic": measured_traffic(name, n, K),
+                   "traffic_source": f"profiles/k_rollout_{name}_{n}.md (rocprofv3 --pmc "
+                                     "FETCH_SIZE, WRITE_SIZE of this command)",
                    "flop_per_substep": fl, "mean_active_rows": r_a,
+                   "mean_active_rows_source": r_a_src,
                    "kernel_ms": kernel_ms,
                    "hbm_algorithmic_gbs": hbm_bytes / (kernel_ms * 1e-3) / 1e9},
       "cpu_baseline": cpu,
@@ -246,18 +291,57 @@ def forward_bench(args, name):
   }
 
 
-def apg_bench(args):
-  """Forward + adjoint of a 128-step rollout of 4096 quadrupeds."""
+def adjoint_cpu_baseline(sys_, q0, qd0, acts, n_frames=N_FRAMES):
+  """fwd+adjoint on the CPU: the torch oracle (oracle/generalized.py,
+  float32, the exact inverse the kernel uses) with torch autograd through
+  the unrolled solver, as the reference differentiates with jax.grad
+  (brax/training/agents/apg/train.py:188-198), on the given sample of the
+  workload's inputs, same loss."""
   import torch
+  from oracle.generalized import Oracle
+  threads, host = host_cpu()
+  torch.set_num_threads(threads)
+  K, n = acts.shape[0], acts.shape[1]
+  o = Oracle(sys_, torch.float32, inverse="exact")
+  t0 = time.perf_counter()
+  a = torch.tensor(acts, dtype=torch.float32, requires_grad=True)
+  qd_init = torch.tensor(qd0, dtype=torch.float32, requires_grad=True)
+  st = o.init(torch.tensor(q0, dtype=torch.float32), qd_init)
+  loss = 1e-3 * (a * a).sum()
+  for t in range(K):
+    st = o.env_step(st, a[t], n_frames)
+    loss = loss - st.qd[:, 0].sum()
+  loss.backward()
+  dt = time.perf_counter() - t0
+  return {
+      "value": n * K / dt, "unit": "env-steps/s", "cores": threads, "kind": "port",
+      "sample": f"{n} quadruped envs x {K} env steps of the workload's inputs, loss and "
+                f"gradients (actions, qd0), torch float32 oracle with autograd, "
+                f"{threads} threads, {dt:.1f} s",
+      "host": host,
+  }
+
+
+def fwd_adjoint_leg(n, T, ckpt, warmup, tune=True, cpu=True):
+  """Forward + adjoint of a T-step rollout of n quadrupeds (loss
+  -sum_t v_torso,x + 1e-3 sum |act|^2, gradient w.r.t. every action and the
+  initial qd, checkpoint every ckpt env steps), timed with HIP events on the
+  stream the kernels run on, beside the forward-only rollout of the same
+  inputs.
+
+  FLOP model of one fwd+adjoint env step: 4 F, F the forward op count of
+  flop_per_substep times the substeps (the forward pass, and 3 F for the
+  gradient: the reverse-mode bound of Griewank and Walther's cheap gradient
+  principle); the checkpoint recompute is not counted."""
+  import torch
+  from brax_amd import _lib
   from brax_amd.generalized import pipeline
 
   dev = torch.device("cuda", 0)
   torch.cuda.set_device(dev)
-  n = args.envs or WORKLOADS["apg"][1]
-  T, ckpt = args.horizon, args.ckpt
   sys_, q0, qd0, rng = workload("apg", n)
-  acts0 = torch.tensor(rng.uniform(-1, 1, (T, n, sys_.act_size())).astype(np.float32),
-                       device=dev)
+  acts_np = rng.uniform(-1, 1, (T, n, sys_.act_size())).astype(np.float32)
+  acts0 = torch.tensor(acts_np, device=dev)
   q0t = torch.tensor(q0, device=dev)
   qd0t = torch.tensor(qd0, device=dev)
 
@@ -273,46 +357,86 @@ def apg_bench(args):
     st = pipeline.State(q=q0t, qd=qd0t, x=None, xd=None)
     return pipeline.rollout(sys_, st, acts0, N_FRAMES, trajectory=True, frames=False)
 
-  from brax_amd import _lib
   epw = [64, 64]
-  if not args.no_tune:
+  if tune:
     from brax_amd.generalized import autotune
     epw[0] = autotune.tune_step(sys_, q0t, qd0t, acts0[:2], N_FRAMES)
-    try:
-      epw[1] = autotune.tune_adjoint(sys_, q0t, qd0t, acts0[:2], N_FRAMES)
-    except _lib.BgxError as e:
-      print(f"apg workload unavailable: {e}", file=sys.stderr)
-      sys.exit(3)
-  try:
+    epw[1] = autotune.tune_adjoint(sys_, q0t, qd0t, acts0[:2], N_FRAMES)
+  fwd_bwd()
+  for _ in range(max(1, warmup // 50)):
     fwd_bwd()
+    fwd_only()
+  torch.cuda.synchronize()
+
+  def timed(fn, reps=3):
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record()
+    for _ in range(reps):
+      out = fn()
+    ev1.record()
+    torch.cuda.synchronize()
+    return out, (time.perf_counter() - t0) / reps, ev0.elapsed_time(ev1) / reps
+
+  _, t_fwd, ms_fwd = timed(fwd_only)
+  (ga, gqd), t_fb, ms_fb = timed(fwd_bwd)
+  finite = bool(torch.isfinite(ga).all() and torch.isfinite(gqd).all())
+  fwd_rate, fb_rate = n * T / t_fwd, n * T / t_fb
+  out = {
+      "value": fb_rate, "unit": "env-steps/s", "envs": n, "horizon": T, "ckpt_every": ckpt,
+      "ms_per_step": t_fb * 1e3 / T,
+      "forward_value": fwd_rate,
+      "fwd_adjoint_over_forward": fb_rate / fwd_rate,
+      "envs_per_workgroup": {"step": epw[0], "adjoint": epw[1]},
+      "finite": finite,
+  }
+  cpu_b, r_a = None, None
+  if cpu:
+    # active rows over this workload's own forward inputs (all n envs, T steps)
+    from oracle.cstep import CStep
+    threads, _ = host_cpu()
+    _, _, r_a = CStep(sys_).rollout(q0, qd0, acts_np, N_FRAMES, threads=threads, rows=True)
+    m = min(n, 4096)
+    cpu_b = adjoint_cpu_baseline(sys_, q0[:m], qd0[:m], acts_np[:2, :m])
+  r_a_src = "oracle/cstep.cpp float32 over this workload's inputs"
+  if r_a is None:
+    r_a, r_a_src = 9.9, "not measured (--no-cpu): 9.9"
+  fl = flop_per_substep(sys_.num_links(), sys_.qd_size(), sys_.num_contacts(),
+                        sys_.num_limit_rows(), len(sys_.geom_type), sys_.solver_iterations, r_a)
+  achieved = 4 * fl * N_FRAMES * n * T / (ms_fb * 1e-3) / 1e12
+  out["roofline"] = {
+      "bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+      "flop_model": "4 x forward op count per env step (forward + reverse-mode bound 3F; "
+                    "recompute not counted)",
+      "flop_per_substep_forward": fl, "mean_active_rows": r_a,
+      "mean_active_rows_source": r_a_src, "gpu_ms_fwd_adjoint": ms_fb,
+      "gpu_ms_forward": ms_fwd,
+  }
+  out["cpu_baseline"] = cpu_b
+  return out
+
+
+def apg_bench(args):
+  """Forward + adjoint of a 128-step rollout of 4096 quadrupeds."""
+  from brax_amd import _lib
+  n = args.envs or WORKLOADS["apg"][1]
+  try:
+    leg = fwd_adjoint_leg(n, args.horizon, args.ckpt, args.warmup, tune=not args.no_tune,
+                          cpu=not args.no_cpu)
   except _lib.BgxError as e:
     print(f"apg workload unavailable: {e}", file=sys.stderr)
     sys.exit(3)
-  for _ in range(max(1, args.warmup // 50)):
-    fwd_bwd()
-    fwd_only()
-  torch.cuda.synchronize()
-  reps = 3
-  t0 = time.perf_counter()
-  for _ in range(reps):
-    fwd_only()
-  torch.cuda.synchronize()
-  t_fwd = (time.perf_counter() - t0) / reps
-  t0 = time.perf_counter()
-  for _ in range(reps):
-    ga, gqd = fwd_bwd()
-  torch.cuda.synchronize()
-  t_fb = (time.perf_counter() - t0) / reps
-  finite = bool(torch.isfinite(ga).all() and torch.isfinite(gqd).all())
-  fwd_rate, fb_rate = n * T / t_fwd, n * T / t_fb
   return {
-      "metric": f"fwd+adjoint env-steps/s @{n} quadrupeds, horizon {T}",
-      "value": fb_rate,
+      "metric": f"fwd+adjoint env-steps/s @{n} quadrupeds, horizon {args.horizon}",
+      "value": leg["value"],
       "unit": "env-steps/s",
       "n_gpus": 1,
-      "steps": T,
+      "steps": args.horizon,
       "warmup": args.warmup,
-      "ms_per_step": t_fb * 1e3 / T,
+      "ms_per_step": leg["ms_per_step"],
       "higher_is_better": True,
       "scaling": "weak",
       "vs_baseline": None,
@@ -320,15 +444,15 @@ def apg_bench(args):
       "data": "synthetic (seeded quadruped states and U(-1,1) actions)",
       "config": {"workload": "differentiable quadruped rollout, loss -sum v_torso,x + "
                              "1e-3 sum |act|^2, gradient w.r.t. actions and qd0, "
-                             f"checkpoint every {ckpt} env steps",
-                 "envs": n, "horizon": T, "ckpt_every": ckpt,
-                 "envs_per_workgroup": {"step": epw[0], "adjoint": epw[1]},
+                             f"checkpoint every {args.ckpt} env steps",
+                 "envs": n, "horizon": args.horizon, "ckpt_every": args.ckpt,
+                 "envs_per_workgroup": leg["envs_per_workgroup"],
                  "parallelism": "none (1 GPU)"},
-      "forward_value": fwd_rate,
-      "fwd_adjoint_over_forward": fb_rate / fwd_rate,
-      "roofline": None,
-      "cpu_baseline": None,
-      "finite": finite,
+      "forward_value": leg["forward_value"],
+      "fwd_adjoint_over_forward": leg["fwd_adjoint_over_forward"],
+      "roofline": leg["roofline"],
+      "cpu_baseline": leg["cpu_baseline"],
+      "finite": leg["finite"],
   }
 
 
@@ -442,6 +566,8 @@ def main():
   ap.add_argument("--ckpt", type=int, default=16)
   ap.add_argument("--no-graph", action="store_true")
   ap.add_argument("--no-cpu", action="store_true")
+  ap.add_argument("--no-adjoint", action="store_true",
+                  help="omit the fwd+adjoint part of the default quadruped line")
   ap.add_argument("--no-tune", action="store_true",
                   help="64 envs per workgroup instead of the autotuner's choice")
   ap.add_argument("--calib-copy", action="store_true",
@@ -460,6 +586,13 @@ def main():
     out = hetero_bench(args)
   else:
     out = forward_bench(args, args.workload)
+    if args.workload == "quadruped" and not args.no_adjoint:
+      # the metric's second part: fwd+adjoint of the 4096-quadruped,
+      # 128-step differentiable rollout with a checkpoint every 16 env steps
+      out["fwd_adjoint"] = fwd_adjoint_leg(4096, 128, 16, args.warmup,
+                                           tune=not args.no_tune, cpu=not args.no_cpu)
+      if out["config"]["envs"] == 65536:
+        out["metric"] = METRIC
   print(json.dumps(out))
 
 

@@ -774,6 +774,10 @@ class Oracle:
     running = torch.ones(N, dtype=torch.bool)
     n_iter = torch.zeros(N, dtype=torch.int64)
     n_trials = torch.zeros(N, dtype=torch.int64)
+    # the discrete decisions of every update (whether it ran, its line-search
+    # trials, the relu pattern of its iterate): the loss is smooth in the
+    # inputs wherever none of them changes
+    decisions = []
 
     def ls_cond(xp, eta_, f_y, g):
       diff = xp - y
@@ -813,7 +817,11 @@ class Oracle:
       error = torch.where(running, err_next, error)
       n_iter = n_iter + running.to(torch.int64)
       n_trials = n_trials + torch.where(running, trials, torch.zeros_like(trials))
-    return x, {"iters": n_iter, "trials": n_trials, "error": error}
+      decisions.append(torch.cat([running[:, None].to(torch.int64),
+                                  torch.where(running, trials, 0)[:, None],
+                                  (rm & (xp > 0)).to(torch.int64)], 1).detach())
+    return x, {"iters": n_iter, "trials": n_trials, "error": error,
+               "decisions": decisions}
 
   def constraint_force(self, st: OState):
     N = st.q.shape[0]

@@ -11,6 +11,11 @@ forms.  Files:
                  the first-substep qf_smooth/qdd; the 1-step state with the
                  reference's Newton-Schulz inverse and its reset flags; the
                  1-step state of the float32 oracle
+  quadruped_ns.npz  the same inputs with the reference's literal
+                 Newton-Schulz inverse over 20 env steps, and the first
+                 substep at which each env's warm start reset
+  quadruped_f32.npz the float32 oracle free-running over the same 20 env
+                 steps
   pendulum3.npz  contact-free triple pendulum, 20 env steps
   ant.npz        ant, 1 env step (float64, and the float32 oracle)
   humanoid.npz   synthetic humanoid, q noise U(-0.01, 0.01) with the pelvis
@@ -109,6 +114,19 @@ def quadruped_ns(path):
                       first_reset=first_reset)
 
 
+def quadruped_f32(path):
+  """The float32 oracle (exact inverse) free-running over the 20 env steps of
+  quadruped.npz's inputs: the state after every env step.  Beside the float64
+  trajectory it shows how far float32 arithmetic of the same algorithm drifts
+  on this contact-rich workload."""
+  sys = S.quadruped(0)
+  g = np.load(os.path.join(HERE, "quadruped.npz"))
+  q_traj, qd_traj = _trajectory(Oracle(sys, torch.float32, inverse="exact"), g["q0"],
+                                g["qd0"], g["acts"], 5)
+  np.savez_compressed(path, q_traj_f32=q_traj.astype(np.float64),
+                      qd_traj_f32=qd_traj.astype(np.float64))
+
+
 def pendulum3(path):
   sys = S.pendulum(3)
   n, steps = 16, 20
@@ -184,6 +202,7 @@ def stages(path):
 if __name__ == "__main__":
   quadruped(os.path.join(HERE, "quadruped.npz"))
   quadruped_ns(os.path.join(HERE, "quadruped_ns.npz"))
+  quadruped_f32(os.path.join(HERE, "quadruped_f32.npz"))
   pendulum3(os.path.join(HERE, "pendulum3.npz"))
   ant(os.path.join(HERE, "ant.npz"))
   humanoid(os.path.join(HERE, "humanoid.npz"))

@@ -30,16 +30,33 @@ def _ratio(gpu, ref):
   return np.abs(gpu - ref) / (ATOL + RTOL * np.abs(ref))
 
 
+# Bound on envs exempt from the band: sorted env by env, the GPU's band
+# ratios may exceed the float32 oracle's (or the band, whichever is larger)
+# by at most this factor.
+K_EXEMPT = 4.0
+
+
+def _dominated(r, r32, k=K_EXEMPT):
+  """Every order statistic of the GPU's per-env ratios r is within k times
+  the float32 oracle's (floored at the band): no exempt env escapes
+  unbounded."""
+  return bool(np.all(np.sort(r) <= k * np.maximum(np.sort(r32), 1.0)))
+
+
 def _one_step_ok(gpu_q, gpu_qd, g):
   """Per-env tolerance check of a 1-step state against the float64 golden.
   Envs where the float32 oracle itself misses the tolerance (a line-search
-  accept/reject decision that float32 rounding flips) are exempt; there must
-  be few of them."""
+  accept/reject decision that float32 rounding flips) are exempt from the
+  band; there must be few of them, and they stay within K_EXEMPT of the
+  float32 oracle's own deviations."""
   r = np.maximum(_ratio(gpu_q, g["q_traj"][0]).max(1), _ratio(gpu_qd, g["qd_traj"][0]).max(1))
   r32 = np.maximum(_ratio(g["q1_f32"], g["q_traj"][0]).max(1),
                    _ratio(g["qd1_f32"], g["qd_traj"][0]).max(1))
   exempt = r32 > 1
+  print(f"one env step: gpu ratios max {r.max():.3g}, float32 oracle max {r32.max():.3g}, "
+        f"exempt {int(exempt.sum())}/{len(r)} (gpu there {np.array2string(r[exempt], precision=3)})")
   assert exempt.sum() <= max(1, len(r) // 8)
+  assert _dominated(r, r32)
   return r[~exempt].max()
 
 
@@ -214,6 +231,75 @@ def test_20_step_horizon_teacher_forced(quad):
         f"median ratio {np.median(ratios):.3f}")
   assert np.median(ratios) < 0.1
   assert miss_gpu <= max(2 * miss_32, int(0.05 * total))
+
+
+# Free-running drift allowance against the float32 oracle: per env step, the
+# GPU's median band ratio over the envs and its count of envs outside the band
+# may exceed the float32 oracle's by this factor (plus a floor of 0.05 on the
+# median and 2 envs on the count).
+K_DRIFT = 3.0
+
+
+def test_20_step_free_running_vs_fp64(quad):
+  """The contact-rich quadruped free-running for 20 env steps, every env step
+  compared with the float64 golden trajectory (quadruped.npz q_traj), beside
+  the float32 oracle free-running from the same inputs (quadruped_f32.npz).
+  Float32 arithmetic of this algorithm drifts from float64 after a few env
+  steps (line-search decisions and stiff contacts amplify rounding), so both
+  drift; the test prints both per step and asserts the GPU drifts at most
+  K_DRIFT times as much as the float32 oracle."""
+  sys, g = quad
+  f32 = np.load(os.path.join(GOLDEN, "quadruped_f32.npz"))
+  st = pipeline.init(sys, _t(g["q0"]), _t(g["qd0"]))
+  _, (qt, qdt) = pipeline.rollout(sys, st, _t(g["acts"]), 5, trajectory=True)
+  torch.cuda.synchronize()
+  qt, qdt = qt.double().cpu().numpy(), qdt.double().cpu().numpy()
+  n = qt.shape[1]
+  tot_gpu = tot_32 = 0
+  for s in range(qt.shape[0]):
+    r_gpu = np.maximum(_ratio(qt[s], g["q_traj"][s]).max(1), _ratio(qdt[s], g["qd_traj"][s]).max(1))
+    r_32 = np.maximum(_ratio(f32["q_traj_f32"][s], g["q_traj"][s]).max(1),
+                      _ratio(f32["qd_traj_f32"][s], g["qd_traj"][s]).max(1))
+    m_gpu, m_32 = int((r_gpu > 1).sum()), int((r_32 > 1).sum())
+    tot_gpu, tot_32 = tot_gpu + m_gpu, tot_32 + m_32
+    dq = np.abs(qt[s] - g["q_traj"][s]).max()
+    print(f"env step {s + 1:2d}: outside the band gpu {m_gpu:2d}/{n} fp32-oracle {m_32:2d}/{n}; "
+          f"median ratio gpu {np.median(r_gpu):.3g} fp32-oracle {np.median(r_32):.3g}; "
+          f"max ratio gpu {r_gpu.max():.3g} fp32-oracle {r_32.max():.3g}; max|dq| gpu {dq:.2e}")
+    assert np.median(r_gpu) <= K_DRIFT * max(np.median(r_32), 0.05), s
+    assert m_gpu <= K_DRIFT * m_32 + 2, s
+    if s == 0:
+      assert r_gpu.max() <= 1.0
+  print(f"env-steps outside the band over 20 steps: gpu {tot_gpu}/{20 * n}, "
+        f"fp32 oracle {tot_32}/{20 * n}")
+  assert tot_gpu <= K_DRIFT * tot_32 + 2
+
+
+def test_newton_schulz_gap(quad):
+  """The gap between the exact inverse (this kernel) and the reference's
+  literal warm-started Newton-Schulz inverse over 20 env steps
+  (quadruped_ns.npz).  Envs whose warm start never reset agree within the
+  band throughout; on envs that reset (the reference's own inverse is then
+  wrong by O(1)) the printed gap is the deviation of the algorithms, not of
+  the arithmetic."""
+  sys, g = quad
+  ns = np.load(os.path.join(GOLDEN, "quadruped_ns.npz"))
+  st = pipeline.init(sys, _t(g["q0"]), _t(g["qd0"]))
+  _, (qt, qdt) = pipeline.rollout(sys, st, _t(g["acts"]), 5, trajectory=True)
+  torch.cuda.synchronize()
+  qt, qdt = qt.double().cpu().numpy(), qdt.double().cpu().numpy()
+  never = ns["first_reset"] < 0
+  assert 8 <= never.sum() < len(never)
+  for s in (0, 19):
+    r = np.maximum(_ratio(qt[s], ns["q_traj_ns"][s]).max(1),
+                   _ratio(qdt[s], ns["qd_traj_ns"][s]).max(1))
+    dq = np.abs(qt[s] - ns["q_traj_ns"][s]).max(1)
+    dqd = np.abs(qdt[s] - ns["qd_traj_ns"][s]).max(1)
+    print(f"after {s + 1:2d} env steps: never-reset envs ({never.sum()}) max ratio "
+          f"{r[never].max():.3g}; reset envs ({(~never).sum()}) max|dq| {dq[~never].max():.3g} "
+          f"max|dqd| {dqd[~never].max():.3g}, median ratio {np.median(r[~never]):.3g}")
+    if s == 0:
+      assert r[never].max() <= 1.0
 
 
 def test_slow_path_many_contacts():
