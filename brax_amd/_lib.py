@@ -50,6 +50,10 @@ SIGNATURES = {
     "bgx_rollout_adjoint": (c_int, [c_vp, c_i32, c_i32, c_i32, c_i32, c_vp,
                                     c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp,
                                     c_vp, c_vp, c_i64, c_vp, c_vp]),
+    "bgx_adjoint_scratch_bytes": (c_i64, []),
+    "bgx_rollout_adjoint_noalloc": (c_int, [c_vp, c_i32, c_i32, c_i32, c_i32, c_vp,
+                                            c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp,
+                                            c_vp, c_vp, c_vp]),
 }
 
 

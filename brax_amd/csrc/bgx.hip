@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <string>
 
 #include "../../include/brax_gen.h"
@@ -294,12 +295,18 @@ int64_t bgx_adjoint_workspace_min_bytes(const bgx_model* m, int32_t n, int32_t c
   return adj_layout(kOps[m->topo], nullptr, n, ckpt_every, n_frames, 1, 1).bytes;
 }
 
-int bgx_rollout_adjoint(const bgx_model* m, int32_t n, int32_t n_steps, int32_t n_frames,
-                        int32_t ckpt_every, const float* ckpt_q, const float* ckpt_qd,
-                        const float* act, int64_t act_step_stride, const float* gq_traj,
-                        const float* gqd_traj, float* gact, float* gq0, float* gqd0,
-                        float* workspace, int64_t workspace_bytes, int32_t* status,
-                        void* stream) {
+}  // extern "C"
+
+namespace {
+
+// The adjoint of a rollout with its workspace at `workspace` (shared by the
+// workspace-argument entry point and the no-allocation one).
+int rollout_adjoint_impl(const bgx_model* m, int32_t n, int32_t n_steps, int32_t n_frames,
+                         int32_t ckpt_every, const float* ckpt_q, const float* ckpt_qd,
+                         const float* act, int64_t act_step_stride, const float* gq_traj,
+                         const float* gqd_traj, float* gact, float* gq0, float* gqd0,
+                         float* workspace, int64_t workspace_bytes, int32_t* status,
+                         void* stream) {
   if (!m) return fail(BGX_ERR_ARG, "model is null");
   if (!valid_topo(m->topo)) return fail(BGX_ERR_TOPOLOGY, "unknown topology id");
   if (n < 0 || n_steps < 0 || n_frames < 1 || ckpt_every < 1)
@@ -406,6 +413,82 @@ int bgx_rollout_adjoint(const bgx_model* m, int32_t n, int32_t n_steps, int32_t 
   hipLaunchKernelGGL(k_cols, grid1((long long)n * nv), dim3(256), 0, s, gqd0, w.adj,
                      (long long)n, nv, nq + nv, nq);
   return check_hip(hipGetLastError(), "output copy");
+}
+
+// Workspace of bgx_rollout_adjoint_noalloc: one array per GPU in this file's
+// device code object, held from the library's first use of a GPU until the
+// process ends.  In host code `scratch` names a host-side stand-in, never the
+// device memory: its device address on each GPU comes from
+// hipGetSymbolAddress (scratch_on_current_device) and only that address is
+// passed to kernels.
+constexpr int64_t kScratchBytes = BGX_ADJOINT_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// the device address of `scratch` on the current device (looked up once per
+// device, with that device current)
+int scratch_on_current_device(float** out) {
+  int dev = -1;
+  int rc = check_hip(hipGetDevice(&dev), "hipGetDevice");
+  if (rc) return rc;
+  if (dev < 0 || dev >= kMaxDevices) return fail(BGX_ERR_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    rc = check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (rc) return rc;
+    if (!p) return fail(BGX_ERR_HIP, "hipGetSymbolAddress(scratch) returned null");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = static_cast<float*>(g_scratch_ptr[dev]);
+  return BGX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bgx_rollout_adjoint(const bgx_model* m, int32_t n, int32_t n_steps, int32_t n_frames,
+                        int32_t ckpt_every, const float* ckpt_q, const float* ckpt_qd,
+                        const float* act, int64_t act_step_stride, const float* gq_traj,
+                        const float* gqd_traj, float* gact, float* gq0, float* gqd0,
+                        float* workspace, int64_t workspace_bytes, int32_t* status,
+                        void* stream) {
+  return rollout_adjoint_impl(m, n, n_steps, n_frames, ckpt_every, ckpt_q, ckpt_qd, act,
+                              act_step_stride, gq_traj, gqd_traj, gact, gq0, gqd0, workspace,
+                              workspace_bytes, status, stream);
+}
+
+int64_t bgx_adjoint_scratch_bytes(void) { return kScratchBytes; }
+
+int bgx_rollout_adjoint_noalloc(const bgx_model* m, int32_t n, int32_t n_steps,
+                                int32_t n_frames, int32_t ckpt_every, const float* ckpt_q,
+                                const float* ckpt_qd, const float* act,
+                                int64_t act_step_stride, const float* gq_traj,
+                                const float* gqd_traj, float* gact, float* gq0, float* gqd0,
+                                int32_t* status, void* stream) {
+  if (!m) return fail(BGX_ERR_ARG, "model is null");
+  if (!valid_topo(m->topo)) return fail(BGX_ERR_TOPOLOGY, "unknown topology id");
+  if (n < 0 || n_steps < 0 || n_frames < 1 || ckpt_every < 1)
+    return fail(BGX_ERR_ARG, "bad env, step, frame or checkpoint count");
+  if (n == 0 || n_steps == 0) return BGX_OK;
+  const int64_t need = adj_layout(kOps[m->topo], nullptr, n, ckpt_every, n_frames, 1, 1).bytes;
+  if (need > kScratchBytes)
+    return fail(BGX_ERR_ARG, "bgx_rollout_adjoint_noalloc: " + std::to_string(n) +
+                                 " envs with a checkpoint every " + std::to_string(ckpt_every) +
+                                 " env steps need " + std::to_string(need) +
+                                 " bytes of workspace; its scratch holds " +
+                                 std::to_string(kScratchBytes) +
+                                 " (use bgx_rollout_adjoint with a workspace)");
+  float* ws = nullptr;
+  int rc = scratch_on_current_device(&ws);
+  if (rc) return rc;
+  return rollout_adjoint_impl(m, n, n_steps, n_frames, ckpt_every, ckpt_q, ckpt_qd, act,
+                              act_step_stride, gq_traj, gqd_traj, gact, gq0, gqd0, ws,
+                              kScratchBytes, status, stream);
 }
 
 }  // extern "C"

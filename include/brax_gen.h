@@ -16,7 +16,9 @@
  * the given HIP stream (NULL = default stream) and return at once; a nonzero
  * return is an error code and bgx_last_error() describes it.  The library
  * holds no per-call workspace and no mutable global state beyond the
- * per-thread error string, so calls on different streams may overlap.
+ * per-thread error string, so calls on different streams may overlap; the
+ * one exception is bgx_rollout_adjoint_noalloc, whose calls share one
+ * scratch array per GPU.
  */
 #ifndef BRAX_GEN_H_
 #define BRAX_GEN_H_
@@ -167,6 +169,29 @@ int bgx_rollout_adjoint(const bgx_model* model, int32_t n_envs, int32_t n_steps,
                         float* gact, float* gq0, float* gqd0, float* workspace,
                         int64_t workspace_bytes, int32_t* status,
                         void* stream);
+
+/* Bytes of the per-GPU scratch array of bgx_rollout_adjoint_noalloc. */
+#define BGX_ADJOINT_SCRATCH_BYTES (128ll << 20)
+int64_t bgx_adjoint_scratch_bytes(void);
+
+/* bgx_rollout_adjoint without a workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation).  No counterpart in the reference, whose
+ * jax.grad allocates through XLA.  Its workspace is a static device array of
+ * BGX_ADJOINT_SCRATCH_BYTES on each GPU the library runs on (held until the
+ * process ends); the call makes no allocation.  Arguments, outputs and
+ * results as bgx_rollout_adjoint (bitwise the same gradients); a problem
+ * whose least workspace (bgx_adjoint_workspace_min_bytes) exceeds the array
+ * returns BGX_ERR_ARG.  Runs on the current device.  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU (other streams or
+ * threads) race and must be ordered by the caller. */
+int bgx_rollout_adjoint_noalloc(const bgx_model* model, int32_t n_envs, int32_t n_steps,
+                                int32_t n_frames, int32_t ckpt_every,
+                                const float* ckpt_q, const float* ckpt_qd,
+                                const float* act, int64_t act_step_stride,
+                                const float* gq_traj, const float* gqd_traj,
+                                float* gact, float* gq0, float* gqd0, int32_t* status,
+                                void* stream);
 
 #ifdef __cplusplus
 }
