@@ -32,6 +32,7 @@ SIGNATURES = {
     "bgx_topology_signature": (ctypes.c_char_p, [c_int]),
     "bgx_topology_dims": (c_int, [c_int, ctypes.POINTER(c_i32)]),
     "bgx_topology_per_env": (c_int, [c_int]),
+    "bgx_topology_tiles": (c_int, [c_int, ctypes.POINTER(c_i32)]),
     "bgx_model_create": (c_int, [c_int, c_vp, c_i64, ctypes.POINTER(c_vp)]),
     "bgx_model_destroy": (c_int, [c_vp]),
     "bgx_model_topology": (c_int, [c_vp]),
@@ -45,11 +46,18 @@ SIGNATURES = {
     "bgx_contacts": (c_int, [c_vp, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "bgx_step_probe": (c_int, [c_vp, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp,
                                c_vp]),
+    "bgx_frames_vjp": (c_int, [c_vp, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                               c_vp]),
     "bgx_adjoint_workspace_bytes": (c_i64, [c_vp, c_i32, c_i32, c_i32]),
     "bgx_adjoint_workspace_min_bytes": (c_i64, [c_vp, c_i32, c_i32, c_i32]),
     "bgx_rollout_adjoint": (c_int, [c_vp, c_i32, c_i32, c_i32, c_i32, c_vp,
                                     c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp,
                                     c_vp, c_vp, c_i64, c_vp, c_vp]),
+    "bgx_batch_create": (c_int, [c_i32, c_vp, c_vp, ctypes.POINTER(c_vp)]),
+    "bgx_batch_destroy": (c_int, [c_vp]),
+    "bgx_batch_sizes": (c_int, [c_vp, c_vp]),
+    "bgx_batch_rollout": (c_int, [c_vp, c_i32, c_i32, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp,
+                                  c_vp, c_vp]),
     "bgx_adjoint_scratch_bytes": (c_i64, []),
     "bgx_rollout_adjoint_noalloc": (c_int, [c_vp, c_i32, c_i32, c_i32, c_i32, c_vp,
                                             c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp,

@@ -1,7 +1,8 @@
 """Build recipe for libbrax_gen.so (gfx950), in-tree.
 
 Regenerates the topology header from the registered systems, compiles
-brax_amd/csrc/bgx_topo.hip once per topology (-DBGX_TOPO=<struct>) and
+brax_amd/csrc/bgx_topo.hip once per tree topology (-DBGX_TOPO=<struct>),
+brax_amd/csrc/bgx_chain.hip (the kernels of every hinge-chain topology) and
 brax_amd/csrc/bgx.hip (the C ABI) in parallel, and links them into
 brax_amd/libbrax_gen.so.
 
@@ -52,9 +53,13 @@ def _run(cmd, verbose):
   return res.stderr
 
 
-def topology_structs(inc_path: str):
+def topology_structs(inc_path: str, macro: str = "BGX_FOR_EACH_TREE_TOPOLOGY"):
+  """The topology structs listed by one X-macro of topologies.inc (by
+  default the tree topologies: one object file each)."""
   text = open(inc_path).read()
-  return re.findall(r"X\(\d+, (Topo_\w+)\)", text)
+  start = text.index(f"#define {macro}(X)")
+  end = text.find("\n\n", start)
+  return re.findall(r"X\(\d+, (Topo_\w+)\)", text[start:end if end >= 0 else None])
 
 
 CACHE = os.path.join(HERE, "_build")
@@ -91,7 +96,8 @@ def build(verbose: bool = False, resource_usage: bool = False, jobs: int = 0,
   extra = ["-Rpass-analysis=kernel-resource-usage"] if resource_usage else []
   jobs = jobs or min(8, (os.cpu_count() or 4), len(structs) + 1)
   os.makedirs(CACHE, exist_ok=True)
-  units = [(os.path.join(CSRC, "bgx.hip"), "bgx", [])]
+  units = [(os.path.join(CSRC, "bgx.hip"), "bgx", []),
+           (os.path.join(CSRC, "bgx_chain.hip"), "bgx_chain", [])]
   for s in structs:
     units.append((os.path.join(CSRC, "bgx_topo.hip"), s, [f"-DBGX_TOPO={s}"]))
   objs, cmds = [], []

@@ -43,6 +43,15 @@ DEFAULT_SLOTS = 6
 # stands on 4-8 foot contacts and its batches leave most of each CU's LDS
 # free, so every candidate gets a slot and the overflow path is not needed
 SLOTS = {"humanoid": 13}
+# other slot counts (LDS tiles) the step kernels are compiled for, per system
+# name, for the autotuner to time: the quadruped trades LDS per env (6 L +
+# 16 KC floats: 4 workgroups per CU up to KC = 6, 3 at 8) against envs
+# taking the slot-per-candidate path (more than KC active contacts)
+SLOT_TILES = {"quadruped": (4, 8)}
+
+# tree systems whose topology gets the persistent batch kernel (k_batch): the
+# heterogeneous workload mixes chains with quadrupeds
+BATCH_SYSTEMS = {"quadruped"}
 
 JAXOPT_TOL = 1e-3  # jaxopt ProximalGradient default tol
 LS_EPS = float(np.finfo(np.float32).eps)
@@ -127,6 +136,20 @@ class Topology:
     # per dof: 1 for a pure rotation axis (motion.vel == 0)
     self.dof_hinge = [int(not np.any(sys.dof.motion.vel[d])) for d in range(self.nv)]
     self.slots = max(1, min(slots, self.ncand)) if self.ncand else 1
+    alt = [k for k in SLOT_TILES.get(sys.name, ()) if 1 <= k <= self.ncand and k != self.slots]
+    self.tiles = (alt + [0, 0])[:2]
+    # a hinge chain (links in series from the world, one motor per hinge,
+    # no contacts, no limits) with per-env tables runs the chain kernels
+    # (bgx_chain.h: several lanes per env) instead of the tree kernels
+    self.chain = (self.per_env and 1 <= self.L <= 32 and
+                  all(t == "1" for t in sys.link_types) and
+                  self.parents == list(range(-1, self.L - 1)) and
+                  self.nq == self.nv == self.nu == self.L and
+                  self.act_q == list(range(self.L)) and self.act_d == list(range(self.L)) and
+                  not self.lim_d and self.ncand == 0 and all(self.dof_hinge))
+    # the persistent heterogeneous-batch kernel is compiled for this tree
+    # topology (its envs can share a launch with chains)
+    self.batch = (not self.chain) and sys.name in BATCH_SYSTEMS
     # LTL factor and limit-row solver vectors in LDS instead of registers:
     # supported, but off for every registered topology (the 16384-humanoid
     # batch ran at 5.2e5 env-steps/s with it against 5.8e5 without: at one
@@ -141,7 +164,8 @@ class Topology:
   def signature(self) -> str:
     key = repr((self.link_types, self.parents, self.act_q, self.act_d,
                 self.lim_d, self.cand_link, self.cand_kind, self.slots,
-                self.plane_z, self.dof_hinge) + ((True,) if self.per_env else ()))
+                self.plane_z, self.dof_hinge) + ((True,) if self.per_env else ()) +
+               ((tuple(self.tiles),) if any(self.tiles) else ()))
     return hashlib.sha1(key.encode()).hexdigest()[:16]
 
   def cpp_struct(self) -> str:
@@ -154,7 +178,9 @@ class Topology:
     s += (f"  static constexpr int L = {self.L}, NQ = {self.nq}, NV = {self.nv}, "
           f"NU = {self.nu}, NLIM = {len(self.lim_d)}, NCAND = {self.ncand}, "
           f"KC = {self.slots}, PLANE_Z = {int(self.plane_z)}, "
-          f"PER_ENV = {int(self.per_env)}, LM_LDS = {int(self.lm_lds)};\n")
+          f"PER_ENV = {int(self.per_env)}, LM_LDS = {int(self.lm_lds)}, "
+          f"CHAIN = {int(self.chain)}, BATCH = {int(self.batch)}, "
+          f"TILE1 = {self.tiles[0]}, TILE2 = {self.tiles[1]};\n")
     s += (f"  static constexpr int OFF_LINK = {self.off_link}, OFF_DOF = "
           f"{self.off_dof}, OFF_ACT = {self.off_act}, OFF_CAND = "
           f"{self.off_cand}, SIZE = {self.size};\n")
@@ -279,6 +305,7 @@ def registered_systems() -> List[Tuple[S.System, bool]]:
       (S.pendulum(1, motor_gear=150.0), False),
       (S.pendulum(3), False),
       (S.humanoid(0), False),
+      (S.humanoid_xml(), False),
   ]
   out += [(S.chain(n, 0), True) for n in CHAIN_LINKS]
   return out
@@ -304,10 +331,14 @@ def write_topology_header(path: str) -> Tuple[str, List[Topology]]:
       t.name = f"{t.name}_{t.signature()[:6]}"
     names.add(t.name)
     out.append(t.cpp_struct())
-  out.append("#define BGX_FOR_EACH_TOPOLOGY(X) \\")
-  for i, t in enumerate(topos):
-    out.append(f"  X({i}, Topo_{t.name}) \\")
-  out.append("")
+  for macro, keep in (("BGX_FOR_EACH_TOPOLOGY", lambda t: True),
+                      ("BGX_FOR_EACH_TREE_TOPOLOGY", lambda t: not t.chain),
+                      ("BGX_FOR_EACH_CHAIN_TOPOLOGY", lambda t: t.chain)):
+    out.append(f"#define {macro}(X) \\")
+    for i, t in enumerate(topos):
+      if keep(t):
+        out.append(f"  X({i}, Topo_{t.name}) \\")
+    out.append("")
   out.append(f"#define BGX_NUM_TOPOLOGIES {len(topos)}")
   text = "\n".join(out) + "\n"
   old = open(path).read() if os.path.exists(path) else None

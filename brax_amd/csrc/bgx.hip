@@ -5,9 +5,11 @@
 // dispatches through the per-topology launch tables.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/brax_gen.h"
 #include "bgx_kernels.h"
@@ -61,6 +63,16 @@ dim3 grid1(long long n) { return dim3((unsigned)((n + 255) / 256)); }
 
 }  // namespace
 
+struct bgx_batch {
+  int kernel_topo;            // topology whose batch kernel runs the batch
+  float* tables;              // device copy of the models' tables, back to back
+  bgx::BatchItem* items;      // device work table, longest items first
+  int n_items;
+  int* counter;               // device [2], zero between launches
+  int grid;                   // persistent workgroups
+  int64_t n_envs, q_floats, qd_floats, act_floats;
+};
+
 struct bgx_model {
   int topo;
   float* table;      // device copy of the model table(s)
@@ -68,6 +80,7 @@ struct bgx_model {
   int maxiter;       // solver iterations (from the table header)
   int64_t n_tables;  // tables (one per env for per-env topologies)
   int epw[2];        // envs per workgroup: step kernels, adjoint kernels (0 = 64)
+  int kc;            // fast contact slots of the step kernels (0: the default)
 };
 
 extern "C" {
@@ -91,6 +104,17 @@ int bgx_topology_dims(int id, int32_t* dims) {
 
 int bgx_topology_per_env(int id) { return valid_topo(id) ? kOps[id]->per_env : -1; }
 
+int bgx_topology_tiles(int id, int32_t* tiles) {
+  if (!valid_topo(id)) return -1;
+  int n = 0;
+  for (int k = 0; k < 3; ++k)
+    if (kOps[id]->tiles[k] > 0) {
+      if (tiles) tiles[n] = kOps[id]->tiles[k];
+      ++n;
+    }
+  return n;
+}
+
 int bgx_model_create(int id, const float* host_table, int64_t n_floats, bgx_model** out) {
   if (!valid_topo(id)) return fail(BGX_ERR_TOPOLOGY, "unknown topology id");
   if (!host_table || !out) return fail(BGX_ERR_ARG, "null argument");
@@ -109,7 +133,8 @@ int bgx_model_create(int id, const float* host_table, int64_t n_floats, bgx_mode
     (void)hipFree(d);
     return rc;
   }
-  *out = new bgx_model{id, d, n_floats, (int)host_table[bgx::H_MAXITER], n_floats / size, {0, 0}};
+  *out = new bgx_model{id, d, n_floats, (int)host_table[bgx::H_MAXITER], n_floats / size, {0, 0},
+                       0};
   return BGX_OK;
 }
 
@@ -124,6 +149,14 @@ int bgx_model_topology(const bgx_model* m) { return m ? m->topo : -1; }
 
 int bgx_model_set_launch(bgx_model* m, int kernels, int32_t envs_per_workgroup) {
   if (!m) return fail(BGX_ERR_ARG, "model is null");
+  if (kernels == BGX_KERNELS_TILE) {
+    const int k = envs_per_workgroup;
+    const int* t = kOps[m->topo]->tiles;
+    if (k != 0 && k != t[0] && k != t[1] && k != t[2])
+      return fail(BGX_ERR_ARG, "the step kernels are not compiled for this slot count");
+    m->kc = k;
+    return BGX_OK;
+  }
   if (kernels != BGX_KERNELS_STEP && kernels != BGX_KERNELS_ADJOINT)
     return fail(BGX_ERR_ARG, "unknown kernel group");
   const int e = envs_per_workgroup;
@@ -134,6 +167,7 @@ int bgx_model_set_launch(bgx_model* m, int kernels, int32_t envs_per_workgroup) 
 }
 
 int bgx_model_get_launch(const bgx_model* m, int kernels) {
+  if (m && kernels == BGX_KERNELS_TILE) return m->kc ? m->kc : kOps[m->topo]->tiles[0];
   if (!m || (kernels != BGX_KERNELS_STEP && kernels != BGX_KERNELS_ADJOINT)) return -1;
   return m->epw[kernels] ? m->epw[kernels] : bgx::WAVE;
 }
@@ -149,7 +183,11 @@ static int launch(const bgx_model* m, bool init_only, bgx::Launch a, void* strea
     return fail(BGX_ERR_ARG, "more envs than per-env model tables");
   a.M = m->table;
   a.epw = m->epw[BGX_KERNELS_STEP];
+  a.kc = m->kc;
   const bgx::TopoOps* ops = kOps[m->topo];
+  a.q_step = (long long)a.n * ops->dims[1], a.qd_step = (long long)a.n * ops->dims[2];
+  if ((a.q_traj || a.qd_traj) && a.n_seg > 1)
+    return fail(BGX_ERR_ARG, "trajectories of a multi-segment launch are not supported");
   hipStream_t s = static_cast<hipStream_t>(stream);
   return check_hip(init_only ? ops->init(a, s) : ops->rollout(a, s), "kernel launch");
 }
@@ -203,6 +241,23 @@ int bgx_contacts(const bgx_model* m, int32_t n, const float* q, float* dist, flo
   a.q = const_cast<float*>(q);
   a.c_dist = dist, a.c_pos = pos, a.c_frame = frame;
   return check_hip(ops->contacts(a, static_cast<hipStream_t>(stream)), "kernel launch");
+}
+
+int bgx_frames_vjp(const bgx_model* m, int32_t n, const float* q, const float* qd,
+                   const float* gx_pos, const float* gx_rot, const float* gxd_ang,
+                   const float* gxd_vel, float* gq, float* gqd, void* stream) {
+  if (!m) return fail(BGX_ERR_ARG, "model is null");
+  if (!valid_topo(m->topo)) return fail(BGX_ERR_TOPOLOGY, "unknown topology id");
+  if (n < 0) return fail(BGX_ERR_ARG, "negative env count");
+  const bgx::TopoOps* ops = kOps[m->topo];
+  if (!ops->frames_vjp)
+    return fail(BGX_ERR_TOPOLOGY, std::string("no frames derivative for topology ") + ops->name);
+  if (n == 0) return BGX_OK;
+  if (!q || !qd || !gq || !gqd) return fail(BGX_ERR_ARG, "null argument");
+  if (ops->per_env && n > m->n_tables)
+    return fail(BGX_ERR_ARG, "more envs than per-env model tables");
+  bgx::FramesVjp a{m->table, n, q, qd, gx_pos, gx_rot, gxd_ang, gxd_vel, gq, gqd};
+  return check_hip(ops->frames_vjp(a, static_cast<hipStream_t>(stream)), "kernel launch");
 }
 
 int bgx_step_probe(const bgx_model* m, int32_t n, int32_t n_frames, float* q, float* qd,
@@ -368,6 +423,7 @@ int rollout_adjoint_impl(const bgx_model* m, int32_t n, int32_t n_steps, int32_t
     f.sub_q = w.sub_q, f.sub_qd = w.sub_qd;
     f.status = status, f.status_or = 1;
     f.epw = m->epw[BGX_KERNELS_ADJOINT];
+    f.kc = m->kc;  // the forward's tile: the recompute repeats its states
     if ((rc = check_hip(ops->rollout(f, s), "recompute launch"))) return rc;
     for (int seg = seg_hi; seg >= seg_lo; --seg) {
       const int s0 = seg * ckpt_every;
@@ -463,6 +519,163 @@ int bgx_rollout_adjoint(const bgx_model* m, int32_t n, int32_t n_steps, int32_t 
 }
 
 int64_t bgx_adjoint_scratch_bytes(void) { return kScratchBytes; }
+
+// ---------------------------------------------------------------------------
+// heterogeneous batches: one persistent launch (bgx_batch.h)
+
+int bgx_batch_create(int32_t n_models, const bgx_model* const* models,
+                     const int32_t* envs_per_model, bgx_batch** out) {
+  if (!out || n_models < 0 || (n_models > 0 && (!models || !envs_per_model)))
+    return fail(BGX_ERR_ARG, "null argument or negative model count");
+  *out = nullptr;
+  int kernel_topo = -1, chain_topo = -1;
+  for (int m = 0; m < n_models; ++m) {
+    const bgx_model* md = models[m];
+    if (!md || !valid_topo(md->topo)) return fail(BGX_ERR_ARG, "null or unknown model");
+    if (envs_per_model[m] < 0) return fail(BGX_ERR_ARG, "negative env count");
+    const bgx::TopoOps* ops = kOps[md->topo];
+    if (ops->per_env && envs_per_model[m] > md->n_tables)
+      return fail(BGX_ERR_ARG, "more envs than per-env model tables");
+    if (ops->chain) {
+      chain_topo = md->topo;
+      continue;
+    }
+    if (!ops->batch)
+      return fail(BGX_ERR_TOPOLOGY, std::string("topology ") + ops->name +
+                                        " has no batch kernel (codegen.BATCH_SYSTEMS)");
+    if (kernel_topo >= 0 && kernel_topo != md->topo)
+      return fail(BGX_ERR_TOPOLOGY, "a batch holds chains and envs of at most one other "
+                                    "topology");
+    kernel_topo = md->topo;
+  }
+  if (kernel_topo < 0) kernel_topo = chain_topo;  // chains only (or empty)
+  // work items: 64 tree envs, or one wavefront of chains (64 / G envs)
+  std::vector<bgx::BatchItem> items;
+  std::vector<int> rank;  // ordering key: tree items first, then wider chains
+  std::vector<int64_t> table_off(n_models), table_len(n_models);
+  int64_t env = 0, qo = 0, qdo = 0, ao = 0, tfloats = 0;
+  for (int m = 0; m < n_models; ++m) {
+    const bgx_model* md = models[m];
+    const bgx::TopoOps* ops = kOps[md->topo];
+    const int nq = ops->dims[1], nv = ops->dims[2], nu = ops->dims[3];
+    const int64_t size = ops->dims[7];
+    const int n = envs_per_model[m];
+    // the model's tables in the batch's table buffer: the first n of a
+    // per-env model, the one shared table otherwise
+    table_off[m] = tfloats;
+    table_len[m] = ops->per_env ? (int64_t)n * size : size;
+    tfloats += table_len[m];
+    const int per = ops->chain ? bgx::WAVE / bgx::chain_lanes(ops->dims[0]) : bgx::WAVE;
+    for (int i = 0; i < n; i += per) {
+      bgx::BatchItem b{};
+      b.m_off = table_off[m] + (ops->per_env ? (int64_t)i * size : 0);
+      b.M_stride = ops->per_env ? (int)size : 0;
+      b.env = env + i;
+      b.q = qo + (int64_t)i * nq, b.qd = qdo + (int64_t)i * nv, b.act = ao + (int64_t)i * nu;
+      b.kind = ops->chain ? bgx::BATCH_CHAIN : bgx::BATCH_TREE;
+      b.n = std::min(per, n - i);
+      b.L = ops->chain ? ops->dims[0] : 0;
+      items.push_back(b);
+      // longest first: a tree wavefront (one env per lane, the whole step
+      // with its contact solve) outlasts every chain wavefront; chains by
+      // lane count, then length
+      rank.push_back(ops->chain ? bgx::chain_lanes(b.L) * 64 + b.L : 1 << 20);
+    }
+    env += n, qo += (int64_t)n * nq, qdo += (int64_t)n * nv, ao += (int64_t)n * nu;
+  }
+  std::vector<size_t> order(items.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](size_t a, size_t b) { return rank[a] > rank[b]; });
+  std::vector<bgx::BatchItem> sorted;
+  for (size_t i : order) sorted.push_back(items[i]);
+  bgx_batch* b = new bgx_batch{};
+  b->kernel_topo = kernel_topo;
+  b->n_items = (int)sorted.size();
+  b->n_envs = env, b->q_floats = qo, b->qd_floats = qdo, b->act_floats = ao;
+  int rc = BGX_OK;
+  if (b->n_items > 0) {
+    const bgx::TopoOps* kops = kOps[kernel_topo];
+    int dev = 0, cus = 0;
+    rc = check_hip(hipGetDevice(&dev), "hipGetDevice");
+    if (!rc)
+      rc = check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
+                     "hipDeviceGetAttribute");
+    if (!rc) {
+      const int64_t resident = (int64_t)kops->batch_occupancy() * std::max(cus, 1);
+      b->grid = (int)std::max<int64_t>(1, std::min<int64_t>(resident, b->n_items));
+      rc = check_hip(hipMalloc(&b->items, sorted.size() * sizeof(bgx::BatchItem)), "hipMalloc");
+    }
+    if (!rc)
+      rc = check_hip(hipMemcpy(b->items, sorted.data(), sorted.size() * sizeof(bgx::BatchItem),
+                               hipMemcpyHostToDevice),
+                     "hipMemcpy");
+    if (!rc) rc = check_hip(hipMalloc(&b->tables, std::max<int64_t>(tfloats, 1) * sizeof(float)),
+                            "hipMalloc");
+    for (int m = 0; m < n_models && !rc; ++m)
+      if (table_len[m] > 0)
+        rc = check_hip(hipMemcpy(b->tables + table_off[m], models[m]->table,
+                                 table_len[m] * sizeof(float), hipMemcpyDeviceToDevice),
+                       "hipMemcpy");
+    if (!rc) rc = check_hip(hipMalloc(&b->counter, 2 * sizeof(int)), "hipMalloc");
+    if (!rc) rc = check_hip(hipMemset(b->counter, 0, 2 * sizeof(int)), "hipMemset");
+    if (!rc) rc = check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  }
+  if (rc) {
+    if (b->items) (void)hipFree(b->items);
+    if (b->tables) (void)hipFree(b->tables);
+    if (b->counter) (void)hipFree(b->counter);
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return BGX_OK;
+}
+
+int bgx_batch_destroy(bgx_batch* b) {
+  if (!b) return BGX_OK;
+  int rc = BGX_OK;
+  if (b->items) rc = check_hip(hipFree(b->items), "hipFree");
+  if (b->tables) {
+    const int rc2 = check_hip(hipFree(b->tables), "hipFree");
+    if (!rc) rc = rc2;
+  }
+  if (b->counter) {
+    const int rc2 = check_hip(hipFree(b->counter), "hipFree");
+    if (!rc) rc = rc2;
+  }
+  delete b;
+  return rc;
+}
+
+int bgx_batch_sizes(const bgx_batch* b, int64_t* sizes) {
+  if (!b || !sizes) return fail(BGX_ERR_ARG, "null argument");
+  sizes[0] = b->n_envs, sizes[1] = b->q_floats, sizes[2] = b->qd_floats;
+  sizes[3] = b->act_floats, sizes[4] = b->n_items, sizes[5] = b->grid;
+  return BGX_OK;
+}
+
+int bgx_batch_rollout(const bgx_batch* b, int32_t n_steps, int32_t n_frames, float* q,
+                      float* qd, const float* act, int64_t act_step_stride, float* q_traj,
+                      float* qd_traj, int32_t* status, void* stream) {
+  if (!b) return fail(BGX_ERR_ARG, "batch is null");
+  if (n_steps < 0 || n_frames < 0) return fail(BGX_ERR_ARG, "negative step count");
+  if (b->n_items == 0) return BGX_OK;
+  if (!q || !qd) return fail(BGX_ERR_ARG, "q/qd must be non-null");
+  if (act && act_step_stride < b->act_floats && n_steps > 1)
+    return fail(BGX_ERR_ARG, "action step stride shorter than a step's actions");
+  bgx::Launch a{};
+  a.M = b->tables;
+  a.items = b->items, a.n_items = b->n_items, a.counter = b->counter;
+  a.n = (int)b->n_envs, a.n_seg = 1;
+  a.n_steps = n_steps, a.n_frames = n_frames;
+  a.q = q, a.qd = qd, a.act = b->act_floats > 0 ? act : nullptr, a.act_stride = act_step_stride;
+  a.q_traj = q_traj, a.qd_traj = qd_traj;
+  a.q_step = b->q_floats, a.qd_step = b->qd_floats;
+  a.status = status;
+  return check_hip(kOps[b->kernel_topo]->batch(a, b->grid, static_cast<hipStream_t>(stream)),
+                   "batch kernel launch");
+}
 
 int bgx_rollout_adjoint_noalloc(const bgx_model* m, int32_t n, int32_t n_steps,
                                 int32_t n_frames, int32_t ckpt_every, const float* ckpt_q,

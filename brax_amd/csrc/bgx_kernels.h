@@ -12,44 +12,14 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "bgx_adjoint.h"
+#include "bgx_batch.h"
+#include "bgx_launch.h"
 #include "bgx_step.h"
 
 namespace bgx {
-
-struct Launch {
-  const float* M;
-  int n;
-  int n_steps;
-  int n_frames;
-  float* q;
-  float* qd;
-  const float* act;
-  long long act_stride;
-  float* q_traj;
-  float* qd_traj;
-  float* x_pos;
-  float* x_rot;
-  float* xd_ang;
-  float* xd_vel;
-  int* status;
-  int status_or;  // OR the status bits into status instead of assigning
-  float* probe;
-  float* sub_q;   // (n_steps * n_frames, n, NQ): state at the start of each substep
-  float* sub_qd;  // (n_steps * n_frames, n, NV)
-  int epw;        // envs (lanes) per workgroup, 1..WAVE; 0 = WAVE
-  // n_seg > 1 (the adjoint's recompute): n_seg independent rollouts of the
-  // same length, blockIdx.y = j: state at q + j n NQ (qd alike), actions
-  // from act + j seg_act_stride, substep states at sub_q + j n_steps
-  // n_frames n NQ (sub_qd alike), status bits ORed atomically
-  int n_seg;
-  long long seg_act_stride;
-  // contacts of the state (bgx_contacts): dist (n, NCAND), pos (n, NCAND, 3),
-  // frame (n, NCAND, 3, 3) rows n, t1, t2
-  float* c_dist;
-  float* c_pos;
-  float* c_frame;
-};
 
 // one adjoint substep: state (q, qd) of substep i, its action, the running
 // cotangent adj (n, NQ + NV) of the substep's output state
@@ -154,69 +124,212 @@ __global__ __launch_bounds__(WAVE) void k_contacts(Launch a) {
   }
 }
 
+// The envs of one tree item, one per lane (a homogeneous launch's
+// workgroup, or a batch item): n_steps x n_frames substeps with the state in
+// registers.  The one call site of the step's substep in the rollout
+// kernel.
 template <class T>
-__global__ __launch_bounds__(WAVE) void k_rollout(Launch a) {
-  __shared__ float smem[lds_floats<T>() * WAVE];  // layout: lds_floats()
-  const int env = blockIdx.x * blockDim.x + threadIdx.x;
-  if (env >= a.n) return;
-  const long long seg = blockIdx.y;
-  float* qp = a.q + seg * a.n * T::NQ;
-  float* qdp = a.qd + seg * a.n * T::NV;
-  const float* actp = a.act ? a.act + seg * a.seg_act_stride : nullptr;
+BGX_D void tree_item(const Launch& a, const BatchItem& b, float* smem) {
+  constexpr int NQ = T::NQ, NV = T::NV, NU = T::NU;
+  const int i = threadIdx.x;
+  if (i >= b.n) return;
+  const long long env = b.env + i;
+  // the stride is the topology's (compile time): a shared table stays one
+  // address for the wavefront, read with scalar loads
+  const float* M = a.M + b.m_off + (T::PER_ENV ? (long long)i * T::SIZE : 0);
+  float* qp = a.q + b.q + (long long)i * NQ;
+  float* qdp = a.qd + b.qd + (long long)i * NV;
+  const float* actp = a.act ? a.act + b.act + (long long)i * NU : nullptr;
+  // substep start states (the adjoint's recompute; homogeneous launches)
   const long long n_sub = (long long)a.n_steps * a.n_frames;
-  float* subq = a.sub_q ? a.sub_q + seg * n_sub * a.n * T::NQ : nullptr;
-  float* subqd = a.sub_qd ? a.sub_qd + seg * n_sub * a.n * T::NV : nullptr;
+  const long long seg = blockIdx.y;
+  float* subq = a.sub_q ? a.sub_q + (seg * n_sub * a.n + env) * NQ : nullptr;
+  float* subqd = a.sub_qd ? a.sub_qd + (seg * n_sub * a.n + env) * NV : nullptr;
   float* lds = smem + threadIdx.x;
-  float q[T::NQ], qd[T::NV];
+  float q[NQ], qd[NV];
 #pragma unroll
-  for (int k = 0; k < T::NQ; ++k) q[k] = qp[(long long)env * T::NQ + k];
+  for (int k = 0; k < NQ; ++k) q[k] = qp[k];
 #pragma unroll
-  for (int k = 0; k < T::NV; ++k) qd[k] = qdp[(long long)env * T::NV + k];
-  float* probe = a.probe ? a.probe + (long long)env * (2 * T::NV + 3) : nullptr;
+  for (int k = 0; k < NV; ++k) qd[k] = qdp[k];
+  float* probe = a.probe ? a.probe + env * (2 * NV + 3) : nullptr;
   int st = 0;
   for (int s = 0; s < a.n_steps; ++s) {
-    float act[T::NU > 0 ? T::NU : 1];
+    float act[NU > 0 ? NU : 1];
 #pragma unroll
-    for (int k = 0; k < T::NU; ++k)
-      act[k] = actp ? actp[s * a.act_stride + (long long)env * T::NU + k] : 0.f;
+    for (int k = 0; k < NU; ++k) act[k] = actp ? actp[s * a.act_stride + k] : 0.f;
     for (int f = 0; f < a.n_frames; ++f) {
       if (subq) {
-        const long long i = (long long)s * a.n_frames + f;
+        const long long o = ((long long)s * a.n_frames + f) * a.n;
 #pragma unroll
-        for (int k = 0; k < T::NQ; ++k) subq[(i * a.n + env) * T::NQ + k] = q[k];
+        for (int k = 0; k < NQ; ++k) subq[o * NQ + k] = q[k];
 #pragma unroll
-        for (int k = 0; k < T::NV; ++k) subqd[(i * a.n + env) * T::NV + k] = qd[k];
+        for (int k = 0; k < NV; ++k) subqd[o * NV + k] = qd[k];
       }
-      st |= substep<T>(env_model<T>(a.M, env), q, qd, act, lds, probe);
+      st |= substep<T>(M, q, qd, act, lds, probe);
     }
     if (a.q_traj) {
 #pragma unroll
-      for (int k = 0; k < T::NQ; ++k)
-        a.q_traj[((long long)s * a.n + env) * T::NQ + k] = q[k];
+      for (int k = 0; k < NQ; ++k) a.q_traj[s * a.q_step + b.q + (long long)i * NQ + k] = q[k];
     }
     if (a.qd_traj) {
 #pragma unroll
-      for (int k = 0; k < T::NV; ++k)
-        a.qd_traj[((long long)s * a.n + env) * T::NV + k] = qd[k];
+      for (int k = 0; k < NV; ++k)
+        a.qd_traj[s * a.qd_step + b.qd + (long long)i * NV + k] = qd[k];
     }
   }
   bool finite = true;
 #pragma unroll
-  for (int k = 0; k < T::NQ; ++k) {
+  for (int k = 0; k < NQ; ++k) {
     finite = finite && isfinite(q[k]);
-    qp[(long long)env * T::NQ + k] = q[k];
+    qp[k] = q[k];
   }
 #pragma unroll
-  for (int k = 0; k < T::NV; ++k) {
+  for (int k = 0; k < NV; ++k) {
     finite = finite && isfinite(qd[k]);
-    qdp[(long long)env * T::NV + k] = qd[k];
+    qdp[k] = qd[k];
   }
   if (!finite) st |= ST_NONFINITE;
-  if (a.x_pos || a.x_rot || a.xd_ang || a.xd_vel) write_frames<T>(a, env, q, qd);
+  if (a.x_pos || a.x_rot || a.xd_ang || a.xd_vel) write_frames<T>(a, (int)env, q, qd);
   if (a.status) {
     if (a.n_seg > 1) atomicOr(a.status + env, st);
     else a.status[env] = a.status_or ? (a.status[env] | st) : st;
   }
+}
+
+template <class T>
+constexpr bool is_chain_kernel() {
+  return std::is_same<T, NoTree>::value;
+}
+
+// LDS of the rollout kernel: the tree step's, or a chain wavefront's, or
+// the larger of the two where the kernel runs both
+template <class T, bool CHAINS>
+constexpr int rollout_lds_floats() {
+  if constexpr (is_chain_kernel<T>()) {
+    return CHAIN_LDS_FLOATS;
+  } else if constexpr (CHAINS) {
+    return lds_floats<T>() * WAVE > CHAIN_LDS_FLOATS ? lds_floats<T>() * WAVE
+                                                      : CHAIN_LDS_FLOATS;
+  } else {
+    return lds_floats<T>() * WAVE;
+  }
+}
+
+// the item of a homogeneous launch's workgroup: its envs (blockIdx.x) of
+// checkpoint segment blockIdx.y
+template <class T>
+BGX_D BatchItem home_item(const Launch& a) {
+  BatchItem b{};
+  const long long seg = blockIdx.y;
+  if constexpr (is_chain_kernel<T>()) {
+    const int L = a.chain_L, size = chain_table_size(L);
+    const int per = (int)blockDim.x / chain_lanes(L);
+    const long long env0 = (long long)blockIdx.x * per;
+    b.kind = BATCH_CHAIN, b.L = L;
+    b.n = (int)(a.n - env0 < per ? a.n - env0 : per);
+    b.env = env0;
+    b.m_off = env0 * size, b.M_stride = size;
+    b.q = env0 * L, b.qd = env0 * L, b.act = env0 * L;
+  } else {
+    const long long env0 = (long long)blockIdx.x * blockDim.x;
+    b.kind = BATCH_TREE;
+    b.n = (int)(a.n - env0 < (long long)blockDim.x ? a.n - env0 : blockDim.x);
+    b.env = env0;
+    b.m_off = T::PER_ENV ? env0 * T::SIZE : 0;
+    b.M_stride = T::PER_ENV ? T::SIZE : 0;
+    b.q = (seg * a.n + env0) * T::NQ;
+    b.qd = (seg * a.n + env0) * T::NV;
+    b.act = seg * a.seg_act_stride + env0 * T::NU;
+  }
+  return b;
+}
+
+// The rollout kernel.  Homogeneous launch (items null): workgroup
+// (blockIdx.x, blockIdx.y) runs its own item once.  Batch launch: a
+// persistent grid; workgroups take items from the counter until the table
+// is exhausted.  T: a tree topology, or NoTree (chains only).  CHAINS: the
+// kernel also runs chain items (the batch kernel of a tree topology); its
+// tree path is the same single call site as the homogeneous kernel's, so
+// both compile it to the same arithmetic.
+template <class T, bool CHAINS>
+__global__ __launch_bounds__(WAVE) void k_rollout(Launch a) {
+  __shared__ float smem[rollout_lds_floats<T, CHAINS>()];
+  const bool batch = a.items != nullptr;
+  for (int round = 0;; ++round) {
+    BatchItem b;
+    if (batch) {
+      const int it = next_item(a);
+      if (it < 0) break;
+      b = a.items[it];
+    } else {
+      if (round > 0) break;
+      b = home_item<T>(a);
+    }
+    if constexpr (CHAINS) {
+      if (b.kind == BATCH_CHAIN) {
+        run_chain_item(a, b, smem);
+        __syncthreads();  // the next item reuses the LDS
+        continue;
+      }
+    }
+    if constexpr (!is_chain_kernel<T>()) tree_item<T>(a, b, smem);
+    __syncthreads();
+  }
+  if (batch) batch_done(a);
+}
+
+// cotangents of the frames of states (q, qd) (pipeline.init's x, xd:
+// kinematics.forward, kinematics.py:31-108) carried to q and qd, as
+// jax.grad differentiates the reference's frames; one thread per (env,
+// input coordinate), coordinate-major so a wavefront shares its coordinate:
+// the kinematics in dual numbers along that coordinate, contracted with the
+// frames' cotangents
+struct FramesVjp {
+  const float* M;
+  int n;
+  const float* q;        // (n, NQ)
+  const float* qd;       // (n, NV)
+  const float* gx_pos;   // (n, L, 3) or null (zero)
+  const float* gx_rot;   // (n, L, 4) or null
+  const float* gxd_ang;  // (n, L, 3) or null
+  const float* gxd_vel;  // (n, L, 3) or null
+  float* gq;             // (n, NQ)
+  float* gqd;            // (n, NV)
+};
+
+template <class T>
+__global__ __launch_bounds__(WAVE) void k_frames_vjp(FramesVjp a) {
+  constexpr int NQ = T::NQ, NV = T::NV, ND = NQ + NV, L = T::L;
+  const int n_pad = (a.n + WAVE - 1) / WAVE * WAVE;
+  const long long g = (long long)blockIdx.x * WAVE + threadIdx.x;
+  const int dir = (int)(g / n_pad), env = (int)(g % n_pad);
+  if (dir >= ND || env >= a.n) return;
+  Dual q[NQ], qd[NV];
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) q[k] = dual(a.q[(long long)env * NQ + k], k == dir ? 1.f : 0.f);
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+    qd[k] = dual(a.qd[(long long)env * NV + k], NQ + k == dir ? 1.f : 0.f);
+  V3T<Dual> xp[L], xa[L], xv[L];
+  Q4T<Dual> xr[L];
+  forward_kinematics<T, true, Dual>(env_model<T>(a.M, env), q, qd, xp, xr, xa, xv);
+  float phi = 0.f;
+  auto dot3 = [](const float* c, V3T<Dual> v) {
+    return c[0] * v.x.d + c[1] * v.y.d + c[2] * v.z.d;
+  };
+#pragma unroll
+  for (int l = 0; l < L; ++l) {
+    const long long o = (long long)env * L + l;
+    if (a.gx_pos) phi += dot3(a.gx_pos + 3 * o, xp[l]);
+    if (a.gx_rot) {
+      const float* c = a.gx_rot + 4 * o;
+      phi += c[0] * xr[l].w.d + c[1] * xr[l].x.d + c[2] * xr[l].y.d + c[3] * xr[l].z.d;
+    }
+    if (a.gxd_ang) phi += dot3(a.gxd_ang + 3 * o, xa[l]);
+    if (a.gxd_vel) phi += dot3(a.gxd_vel + 3 * o, xv[l]);
+  }
+  if (dir < NQ) a.gq[(long long)env * NQ + dir] = phi;
+  else a.gqd[(long long)env * NV + dir - NQ] = phi;
 }
 
 // adjoint stage A1: the substep with a slot per candidate, recording its
@@ -296,6 +409,33 @@ __global__ __launch_bounds__(WAVE) void k_adj_b(AdjLaunch a) {
   a.adj[(long long)env * ND + dir] = r;
 }
 
+// a batch launch of the rollout kernel: a persistent grid of `grid`
+// one-wavefront workgroups (a.items set)
+template <class T>
+hipError_t launch_batch(const Launch& a, int grid, hipStream_t s) {
+  hipLaunchKernelGGL((k_rollout<T, true>), dim3((unsigned)grid), dim3(WAVE), 0, s, a);
+  return hipGetLastError();
+}
+
+// resident workgroups of the batch kernel per compute unit
+template <class T>
+int batch_blocks_per_cu() {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_rollout<T, true>, WAVE, 0) !=
+      hipSuccess)
+    return 1;
+  return nb > 0 ? nb : 1;
+}
+
+// A tree topology with K fast contact slots in place of its default KC: the
+// LDS tile of the step kernel (6 L + 16 K floats per lane).  Every template
+// of the step reads the slot count as T::KC, so Tile<T, K> instantiates the
+// same code with the other tile.
+template <class T, int K>
+struct Tile : T {
+  static constexpr int KC = K;
+};
+
 // per-topology host entry points
 
 struct TopoOps {
@@ -310,6 +450,14 @@ struct TopoOps {
   hipError_t (*adj_rev)(const AdjLaunch&, hipStream_t);
   hipError_t (*adj_b)(const AdjLaunch&, hipStream_t);
   hipError_t (*contacts)(const Launch&, hipStream_t);
+  hipError_t (*frames_vjp)(const FramesVjp&, hipStream_t);
+  int chain;     // a hinge-chain topology (bgx_chain.h kernels)
+  int tiles[3];  // fast contact slots the step kernels are compiled for (0: none)
+  // the persistent batch kernel that runs this topology's envs beside
+  // chains (null: the topology cannot join a heterogeneous batch), and its
+  // resident workgroups per compute unit
+  hipError_t (*batch)(const Launch&, int grid, hipStream_t);
+  int (*batch_occupancy)();
 };
 
 template <class T>
@@ -319,12 +467,24 @@ struct Ops {
     hipLaunchKernelGGL(k_init<T>, env_grid(a.n, b), dim3(b), 0, s, a);
     return hipGetLastError();
   }
-  static hipError_t rollout(const Launch& a, hipStream_t s) {
+  template <class TT>
+  static hipError_t rollout_tile(const Launch& a, hipStream_t s) {
     const int b = block_lanes(a.epw);
     const dim3 g = env_grid(a.n, b);
-    hipLaunchKernelGGL(k_rollout<T>, dim3(g.x, a.n_seg > 1 ? (unsigned)a.n_seg : 1u), dim3(b),
-                       0, s, a);
+    Launch h = a;
+    h.items = nullptr;
+    hipLaunchKernelGGL((k_rollout<TT, false>), dim3(g.x, a.n_seg > 1 ? (unsigned)a.n_seg : 1u),
+                       dim3(b), 0, s, h);
     return hipGetLastError();
+  }
+  static hipError_t rollout(const Launch& a, hipStream_t s) {
+    if constexpr (T::TILE1 > 0) {
+      if (a.kc == T::TILE1) return rollout_tile<Tile<T, T::TILE1>>(a, s);
+    }
+    if constexpr (T::TILE2 > 0) {
+      if (a.kc == T::TILE2) return rollout_tile<Tile<T, T::TILE2>>(a, s);
+    }
+    return rollout_tile<T>(a, s);
   }
   static hipError_t adj_fwd(const AdjLaunch& a, hipStream_t s) {
     const int b = block_lanes(a.epw);
@@ -345,6 +505,12 @@ struct Ops {
                        0, s, a);
     return hipGetLastError();
   }
+  static hipError_t frames_vjp(const FramesVjp& a, hipStream_t s) {
+    const long long threads = (long long)(a.n + WAVE - 1) / WAVE * WAVE * (T::NQ + T::NV);
+    hipLaunchKernelGGL(k_frames_vjp<T>, dim3((unsigned)((threads + WAVE - 1) / WAVE)),
+                       dim3(WAVE), 0, s, a);
+    return hipGetLastError();
+  }
   static hipError_t contacts(const Launch& a, hipStream_t s) {
     const int b = block_lanes(a.epw);
     hipLaunchKernelGGL(k_contacts<T>, env_grid(a.n, b), dim3(b), 0, s, a);
@@ -363,6 +529,9 @@ struct Ops {
               nullptr,
               nullptr};
     if constexpr (T::NCAND > 0) t.contacts = &contacts;
+    t.frames_vjp = &frames_vjp;
+    t.tiles[0] = T::KC, t.tiles[1] = T::TILE1, t.tiles[2] = T::TILE2;
+    if constexpr (T::BATCH) t.batch = &launch_batch<T>, t.batch_occupancy = &batch_blocks_per_cu<T>;
     // the adjoint is compiled for topologies with a shared model table
     if constexpr (!T::PER_ENV) t.adj_fwd = &adj_fwd, t.adj_rev = &adj_rev, t.adj_b = &adj_b;
     return t;

@@ -15,6 +15,7 @@ Also provided, for the callers of the reference's step:
 
 from __future__ import annotations
 
+import collections
 import ctypes
 from typing import Dict, Optional, Tuple
 
@@ -90,12 +91,16 @@ class DeviceModel:
         pass
 
 
-_MODELS: Dict[Tuple[int, str], Tuple[S.System, DeviceModel]] = {}
+# uploaded models, most recently used last; at most _MODELS_MAX are kept
+# (a dropped model's device table is freed when its last user lets go)
+_MODELS: "collections.OrderedDict[Tuple[int, str], Tuple[S.System, DeviceModel]]" = (
+    collections.OrderedDict())
+_MODELS_MAX = 256
 
 
 def device_model(sys, device=None) -> DeviceModel:
   """The uploaded model of a System or SystemBatch (cached per object and
-  device)."""
+  device, least recently used first out)."""
   device = torch.device(device if device is not None else "cuda")
   if device.type != "cuda":
     raise ValueError("the generalized pipeline runs on the GPU only")
@@ -108,6 +113,9 @@ def device_model(sys, device=None) -> DeviceModel:
       S.from_arrays_check(s)
     hit = (sys, DeviceModel(sys, device))
     _MODELS[key] = hit
+    while len(_MODELS) > _MODELS_MAX:
+      _MODELS.popitem(last=False)
+  _MODELS.move_to_end(key)
   return hit[1]
 
 
@@ -165,13 +173,61 @@ def _with_contacts(sys, state: State, debug: bool) -> State:
   return state.replace(contact=contacts(sys, state.q)) if debug else state
 
 
+class _FramesFn(torch.autograd.Function):
+  """Frames x, xd of states (q, qd) (bgx_init), differentiable: the backward
+  pass is bgx_frames_vjp (the reference's State.x, State.xd carry jax.grad
+  gradients back to q and qd)."""
+
+  @staticmethod
+  def forward(ctx, q, qd, m):
+    qc, qdc = q.detach().contiguous(), qd.detach().contiguous()
+    n = qc.shape[0]
+    xp, xr, xa, xv = _frames(m, n, m.device)
+    lib = _lib.load()
+    _lib.check(lib.bgx_init(m.handle, n, _lib.ptr(qc), _lib.ptr(qdc), _lib.ptr(xp),
+                            _lib.ptr(xr), _lib.ptr(xa), _lib.ptr(xv), _stream()))
+    ctx.save_for_backward(qc, qdc)
+    ctx.m = m
+    return xp, xr, xa, xv
+
+  @staticmethod
+  def backward(ctx, gxp, gxr, gxa, gxv):
+    qc, qdc = ctx.saved_tensors
+    m = ctx.m
+    n = qc.shape[0]
+    gq = torch.empty(n, m.nq, device=m.device)
+    gqd = torch.empty(n, m.nv, device=m.device)
+    c = lambda g: None if g is None else g.contiguous().float()  # noqa: E731
+    gxp, gxr, gxa, gxv = c(gxp), c(gxr), c(gxa), c(gxv)
+    lib = _lib.load()
+    _lib.check(lib.bgx_frames_vjp(m.handle, n, _lib.ptr(qc), _lib.ptr(qdc), _lib.ptr(gxp),
+                                  _lib.ptr(gxr), _lib.ptr(gxa), _lib.ptr(gxv), _lib.ptr(gq),
+                                  _lib.ptr(gqd), _stream()))
+    return gq, gqd, None
+
+
+def frames(sys, q, qd):
+  """(x, xd) of states q (N, q_size), qd (N, qd_size): kinematics.forward
+  (brax/kinematics.py:31-108) as pipeline.init computes them; differentiable
+  with respect to q and qd."""
+  m = device_model(sys, q.device)
+  xp, xr, xa, xv = _FramesFn.apply(q, qd, m)
+  return Transform(xp, xr), Motion(xa, xv)
+
+
 def init(sys: S.System, q, qd, unused_act=None, unused_ctrl=None,
          debug: bool = False, device=None) -> State:
-  """pipeline.init (brax/generalized/pipeline.py:32-61)."""
+  """pipeline.init (brax/generalized/pipeline.py:32-61).  When q or qd
+  require gradients, the frames x, xd carry them (bgx_frames_vjp)."""
   del unused_act, unused_ctrl
   if device is None:
     device = q.device if torch.is_tensor(q) and q.is_cuda else "cuda"
   m = device_model(sys, device)
+  if _needs_grad(q, qd):
+    qb = _as_batch(q, m.nq, m.device, "q")
+    qdb = _as_batch(qd, m.nv, m.device, "qd")
+    x, xd = frames(sys, qb, qdb)
+    return _with_contacts(sys, State(q=qb, qd=qdb, x=x, xd=xd), debug)
   qb = _as_batch(q, m.nq, m.device, "q").clone()
   qdb = _as_batch(qd, m.nv, m.device, "qd").clone()
   n = qb.shape[0]
@@ -205,9 +261,10 @@ def pipeline_step(sys: S.System, state: State, act, n_frames: int = 1,
 
   Differentiable like the reference's step under jax.grad: when q, qd or
   act require gradients, the new q and qd carry them through the
-  hand-written adjoint (a one-step ``rollout_grad``); the frames x, xd of
-  the returned State do not.  With debug=True the State carries the
-  contacts of the new state (``contacts``).
+  hand-written adjoint (a one-step ``rollout_grad``), and the frames x, xd
+  of the returned State carry them on to q and qd (``frames``).  With
+  debug=True the State carries the contacts of the new state
+  (``contacts``).
   """
   m = device_model(sys, state.q.device)
   if _needs_grad(state.q, state.qd, act):
@@ -219,13 +276,8 @@ def pipeline_step(sys: S.System, state: State, act, n_frames: int = 1,
       a = torch.zeros(1, n, 0, device=m.device)
     qt, qdt, status = rollout_grad(sys, state.q, state.qd, a, n_frames, ckpt_every=1)
     q, qd = qt[0], qdt[0]
-    xp, xr, xa, xv = _frames(m, n, m.device)
-    lib = _lib.load()
-    qc, qdc = q.detach().contiguous(), qd.detach().contiguous()
-    _lib.check(lib.bgx_init(m.handle, n, _lib.ptr(qc), _lib.ptr(qdc), _lib.ptr(xp),
-                            _lib.ptr(xr), _lib.ptr(xa), _lib.ptr(xv), _stream()))
-    return _with_contacts(sys, State(q=q, qd=qd, x=Transform(xp, xr), xd=Motion(xa, xv),
-                                     status=status), debug)
+    x, xd = frames(sys, q, qd)
+    return _with_contacts(sys, State(q=q, qd=qd, x=x, xd=xd, status=status), debug)
   q = state.q.contiguous().clone()
   qd = state.qd.contiguous().clone()
   n = q.shape[0]

@@ -1013,6 +1013,82 @@ def ant() -> System:
   return b.build()
 
 
+def humanoid_xml() -> System:
+  """The reference's humanoid, brax/envs/assets/humanoid.xml, transcribed.
+
+  The same compiler calls load_mjcf makes for that file (bodies, joints,
+  geoms and motors in document order, its defaults: joint armature 1,
+  damping 1, limited; geoms contype 0, conaffinity 0; motor ctrlrange
+  +-0.4; options timestep 0.003, 8 iterations, matrix_inv_iterations 20,
+  solver_maxls 15), so that the GPU tests, which cannot read the reference
+  tree, step the reference's own model; tests/test_mjcf.py checks every
+  array against load_mjcf of the file.  After fusing the jointless feet: 11
+  links, 23 dofs, 17 motors, 17 limit rows and 2 foot-sphere contacts.
+  """
+  deg = np.pi / 180.0
+  b = ModelBuilder("humanoid_xml", timestep=0.003, iterations=8, solver_maxls=15,
+                   matrix_inv_iterations=20)
+  g = dict(contype=0, conaffinity=0)
+  b.add_geom(0, GEOM_PLANE, (20, 20, 0.125), contype=0, conaffinity=1, friction=(1.0, 0.1, 0.1),
+             name="floor")
+
+  def hinge(body, name, axis, rng, armature, damping=1.0, stiffness=0.0, pos=(0, 0, 0)):
+    b.add_hinge(body, name, axis, pos=pos, range_rad=(rng[0] * deg, rng[1] * deg),
+                armature=armature, damping=damping, stiffness=stiffness)
+
+  torso = b.add_body("torso", 0, pos=(0, 0, 1.4))
+  b.add_free_joint(torso, "root")
+  b.add_geom(torso, GEOM_CAPSULE, (0.07,), fromto=(0, -.07, 0, 0, .07, 0), name="torso1", **g)
+  b.add_geom(torso, GEOM_SPHERE, (.09,), pos=(0, 0, .19), name="head", **g)
+  b.add_geom(torso, GEOM_CAPSULE, (0.06,), fromto=(-.01, -.06, -.12, -.01, .06, -.12),
+             name="uwaist", **g)
+  lwaist = b.add_body("lwaist", torso, pos=(-.01, 0, -0.260), quat=(1.000, 0, -0.002, 0))
+  b.add_geom(lwaist, GEOM_CAPSULE, (0.06,), fromto=(0, -.06, 0, 0, .06, 0), name="lwaist", **g)
+  hinge(lwaist, "abdomen_z", (0, 0, 1), (-45, 45), 0.02, 5, 20, pos=(0, 0, 0.065))
+  hinge(lwaist, "abdomen_y", (0, 1, 0), (-75, 30), 0.02, 5, 10, pos=(0, 0, 0.065))
+  pelvis = b.add_body("pelvis", lwaist, pos=(0, 0, -0.165), quat=(1.000, 0, -0.002, 0))
+  hinge(pelvis, "abdomen_x", (1, 0, 0), (-35, 35), 0.02, 5, 10, pos=(0, 0, 0.1))
+  b.add_geom(pelvis, GEOM_CAPSULE, (0.09,), fromto=(-.02, -.07, 0, -.02, .07, 0), name="butt",
+             **g)
+  for side, sy, hx, hz, hy_arm, knee_stiff in (("right", -1, (1, 0, 0), (0, 0, 1), 0.0080, 0.0),
+                                               ("left", 1, (-1, 0, 0), (0, 0, -1), 0.01, 1.0)):
+    thigh = b.add_body(f"{side}_thigh", pelvis, pos=(0, 0.1 * sy, -0.04))
+    hinge(thigh, f"{side}_hip_x", hx, (-25, 5), 0.01, 5, 10)
+    hinge(thigh, f"{side}_hip_z", hz, (-60, 35), 0.01, 5, 10)
+    hinge(thigh, f"{side}_hip_y", (0, 1, 0), (-110, 20), hy_arm, 5, 20)
+    b.add_geom(thigh, GEOM_CAPSULE, (0.06,), fromto=(0, 0, 0, 0, -0.01 * sy, -.34),
+               name=f"{side}_thigh1", **g)
+    shin = b.add_body(f"{side}_shin", thigh, pos=(0, -0.01 * sy, -0.403))
+    hinge(shin, f"{side}_knee", (0, -1, 0), (-160, -2), 0.0060, stiffness=knee_stiff,
+          pos=(0, 0, .02))
+    b.add_geom(shin, GEOM_CAPSULE, (0.049,), fromto=(0, 0, 0, 0, 0, -.3), name=f"{side}_shin1",
+               **g)
+    foot = b.add_body(f"{side}_foot", shin, pos=(0, 0, -0.45))
+    b.add_geom(foot, GEOM_SPHERE, (0.075,), pos=(0, 0, 0.1), contype=1, conaffinity=0,
+               name=f"{side}_foot")
+  for side, sy, s1, s2, rng, el in (("right", -1, (2, 1, 1), (0, -1, 1), (-85, 60), (0, -1, 1)),
+                                    ("left", 1, (2, -1, 1), (0, 1, 1), (-60, 85), (0, -1, -1))):
+    upper = b.add_body(f"{side}_upper_arm", torso, pos=(0, 0.17 * sy, 0.06))
+    hinge(upper, f"{side}_shoulder1", s1, rng, 0.0068, stiffness=1)
+    hinge(upper, f"{side}_shoulder2", s2, rng, 0.0051, stiffness=1)
+    b.add_geom(upper, GEOM_CAPSULE, (0.04, 0.16), fromto=(0, 0, 0, .16, .16 * sy, -.16),
+               name=f"{side}_uarm1", **g)
+    lower = b.add_body(f"{side}_lower_arm", upper, pos=(.18, .18 * sy, -.18))
+    hinge(lower, f"{side}_elbow", el, (-90, 50), 0.0028, stiffness=0)
+    b.add_geom(lower, GEOM_CAPSULE, (0.031,), fromto=(0.01, -0.01 * sy, 0.01, .17, -.17 * sy, .17),
+               name=f"{side}_larm", **g)
+    b.add_geom(lower, GEOM_SPHERE, (0.04,), pos=(.18, -.18 * sy, .18), name=f"{side}_hand", **g)
+  for joint, gear in (("abdomen_y", 100), ("abdomen_z", 100), ("abdomen_x", 100),
+                      ("right_hip_x", 100), ("right_hip_z", 100), ("right_hip_y", 300),
+                      ("right_knee", 200), ("left_hip_x", 100), ("left_hip_z", 100),
+                      ("left_hip_y", 300), ("left_knee", 200), ("right_shoulder1", 25),
+                      ("right_shoulder2", 25), ("right_elbow", 25), ("left_shoulder1", 25),
+                      ("left_shoulder2", 25), ("left_elbow", 25)):
+    b.add_motor(joint, float(gear), (-.4, .4))
+  b.bodies = _fuse_jointless(b)
+  return b.build()
+
+
 def pendulum(n_links: int = 1, motor_gear: Optional[float] = None,
              ctrlrange=(-100.0, 100.0), timestep: float = 0.001,
              position_servo: Optional[Tuple[float, float, Tuple[float, float]]] = None

@@ -64,6 +64,10 @@ int bgx_topology_dims(int topo_id, int32_t* dims);
 /* 1 if the topology takes one model table per env (bgx_model_create with
  * k tables for up to k envs), 0 if one table is shared, -1 if unknown. */
 int bgx_topology_per_env(int topo_id);
+/* The fast-contact-slot counts (LDS tiles) the topology's step kernels are
+ * compiled for, default first, into tiles (up to 3 entries); returns the
+ * count, -1 if the id is unknown. */
+int bgx_topology_tiles(int topo_id, int32_t* tiles);
 
 /* Uploads a model table (float32, layout of brax_amd/codegen.py pack_model)
  * for a topology; for a per-env topology, k tables back to back (env e uses
@@ -83,6 +87,12 @@ int bgx_model_topology(const bgx_model* model);
  * autotuner (brax_amd/generalized/autotune.py) sets it. */
 #define BGX_KERNELS_STEP 0
 #define BGX_KERNELS_ADJOINT 1
+/* kernels = BGX_KERNELS_TILE sets the step kernels' LDS tile instead: the
+ * fast contact slots per env (one of bgx_topology_tiles; 0 = the default,
+ * dims[6]); the autotuner times the compiled tiles.  Results agree within
+ * rounding across tiles (an env with more active contacts than slots takes
+ * the slot-per-candidate path). */
+#define BGX_KERNELS_TILE 2
 int bgx_model_set_launch(bgx_model* model, int kernels, int32_t envs_per_workgroup);
 /* the current setting (64 for 0), or -1 on bad arguments */
 int bgx_model_get_launch(const bgx_model* model, int kernels);
@@ -123,6 +133,16 @@ int bgx_rollout(const bgx_model* model, int32_t n_envs, int32_t n_steps,
  * the topology's order.  A topology without candidates writes nothing. */
 int bgx_contacts(const bgx_model* model, int32_t n_envs, const float* q, float* dist,
                  float* pos, float* frame, void* stream);
+
+/* Reverse-mode derivative of pipeline.init's frames (kinematics.forward,
+ * brax/kinematics.py:31-108), as jax.grad differentiates the reference's
+ * State.x / State.xd: gq (N, NQ) and gqd (N, NV) receive the cotangents of
+ * q and qd given those of x_pos (N, L, 3), x_rot (N, L, 4), xd_ang (N, L, 3)
+ * and xd_vel (N, L, 3) (each NULL = zero) at the states (q, qd).  Hinge-chain
+ * topologies return BGX_ERR_TOPOLOGY. */
+int bgx_frames_vjp(const bgx_model* model, int32_t n_envs, const float* q, const float* qd,
+                   const float* gx_pos, const float* gx_rot, const float* gxd_ang,
+                   const float* gxd_vel, float* gq, float* gqd, void* stream);
 
 /* Diagnostic variant of bgx_step: probe (N, 2 * NV + 3) receives, for the
  * last substep, qf_smooth (dynamics.forward), qdd (integrator), and the
@@ -169,6 +189,36 @@ int bgx_rollout_adjoint(const bgx_model* model, int32_t n_envs, int32_t n_steps,
                         float* gact, float* gq0, float* gqd0, float* workspace,
                         int64_t workspace_bytes, int32_t* status,
                         void* stream);
+
+/* A heterogeneous batch: envs of several models stepped by one persistent
+ * kernel launch, replacing the reference's vmap over per-env System arrays
+ * (brax/envs/wrappers/training.py:223-260; one compiled program per
+ * structure there).  models[m] contributes envs_per_model[m] envs (a per-env
+ * model takes its first tables); the models may be hinge chains of any
+ * length (per-env tables) and models of at most one other topology, which
+ * must have a batch kernel.  The batch's arrays are packed model after
+ * model, env after env: q holds sizes[1] floats (model m's envs as
+ * (n_m, NQ_m) rows), qd sizes[2], a step's actions sizes[3].  Creation
+ * allocates and uploads the device work table (synchronously).  sizes
+ * receives {envs, q floats, qd floats, action floats per step, work items,
+ * workgroups}. */
+typedef struct bgx_batch bgx_batch;
+int bgx_batch_create(int32_t n_models, const bgx_model* const* models,
+                     const int32_t* envs_per_model, bgx_batch** out);
+int bgx_batch_destroy(bgx_batch* batch);
+int bgx_batch_sizes(const bgx_batch* batch, int64_t* sizes);
+
+/* n_steps env steps of n_frames substeps of every env of the batch in one
+ * launch (bgx_rollout for a batch): q, qd (packed) updated in place; step t
+ * applies act + t * act_step_stride (packed, NULL = zero action);
+ * q_traj (n_steps, q floats) and qd_traj receive the state after every env
+ * step when non-NULL; status (envs) the BGX_STATUS_NONFINITE bit per env.
+ * Each env's results are bitwise those of bgx_rollout on its model alone.
+ * The batch's work counter is shared by its launches: launches of one batch
+ * must not overlap (one stream).  Graph-capturable. */
+int bgx_batch_rollout(const bgx_batch* batch, int32_t n_steps, int32_t n_frames, float* q,
+                      float* qd, const float* act, int64_t act_step_stride, float* q_traj,
+                      float* qd_traj, int32_t* status, void* stream);
 
 /* Bytes of the per-GPU scratch array of bgx_rollout_adjoint_noalloc. */
 #define BGX_ADJOINT_SCRATCH_BYTES (128ll << 20)

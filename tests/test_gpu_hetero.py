@@ -125,29 +125,106 @@ def test_chain_golden_20_steps(n_links):
   assert worst <= 1.0
 
 
+def _homogeneous(sys, quad, q0, qd0, acts):
+  """One env alone: a shared-table model for the quadruped, a one-env
+  SystemBatch for a chain (the kernels of pipeline.rollout)."""
+  obj = pipeline.SystemBatch([sys]) if sys is not quad else quad
+  st = pipeline.init(obj, torch.tensor(q0[None], device=DEV), torch.tensor(qd0[None], device=DEV))
+  st, (qt, qdt) = pipeline.rollout(obj, st, torch.tensor(acts[:, None], device=DEV), 5,
+                                   trajectory=True)
+  torch.cuda.synchronize()
+  return st.q[0].cpu().numpy(), st.qd[0].cpu().numpy(), qt[:, 0].cpu().numpy(), qdt[:, 0].cpu().numpy()
+
+
 def test_hetero_batch_matches_homogeneous_runs():
+  """One persistent launch over quadrupeds and chains of several lengths:
+  every env's final state and trajectory bitwise equal a run of its own."""
   rng = np.random.default_rng(11)
   quad = S.quadruped(0)
-  chains = [S.chain(L, s) for s, L in enumerate([2, 5, 13, 32, 5, 32])]
-  systems = [quad] * 8 + chains
+  chains = [S.chain(L, s) for s, L in enumerate([2, 5, 13, 32, 5, 32, 3, 17, 9])]
+  systems = [quad] * 70 + chains
   perm = rng.permutation(len(systems))
   systems = [systems[i] for i in perm]
   hb = hetero.HeteroBatch(systems, DEV)
-  n, T = len(systems), 4
+  assert len(hb.groups) == 8  # quadrupeds share one table; a group per chain length
+  T = 4
   q0 = [(s.init_q + rng.uniform(-0.1, 0.1, s.q_size())).astype(np.float32) for s in systems]
   qd0 = [(0.1 * rng.standard_normal(s.qd_size())).astype(np.float32) for s in systems]
   acts = [rng.uniform(-1, 1, (T, s.act_size())).astype(np.float32) for s in systems]
-  q = hb.split([torch.tensor(x) for x in q0])
-  qd = hb.split([torch.tensor(x) for x in qd0])
-  ag = [torch.stack([torch.tensor(acts[e]) for e in g.envs], 1).to(DEV).contiguous()
-        for g in hb.groups]
-  hb.rollout(q, qd, ag, T, 5)
+  q, qd, ap = hb.pack(q0, "q"), hb.pack(qd0, "qd"), hb.pack(acts, "act")
+  status, qt, qdt = hb.rollout(q, qd, ap, T, 5, trajectory=True)
   torch.cuda.synchronize()
+  assert int(status.abs().sum()) == 0
   for e, s in enumerate(systems):
-    obj = pipeline.SystemBatch([s]) if s is not quad else quad
-    st = pipeline.init(obj, torch.tensor(q0[e][None], device=DEV),
-                       torch.tensor(qd0[e][None], device=DEV))
-    st, _ = pipeline.rollout(obj, st, torch.tensor(acts[e][:, None], device=DEV), 5)
+    rq, rqd, rqt, rqdt = _homogeneous(s, quad, q0[e], qd0[e], acts[e])
+    sq, sqd, _ = hb.env_rows(e)
+    np.testing.assert_array_equal(q[sq].cpu().numpy(), rq)
+    np.testing.assert_array_equal(qd[sqd].cpu().numpy(), rqd)
+    np.testing.assert_array_equal(qt[:, sq].cpu().numpy(), rqt)
+    np.testing.assert_array_equal(qdt[:, sqd].cpu().numpy(), rqdt)
+
+
+def test_hetero_batch_full_size():
+  """The heterogeneous workload at its own size: 32768 envs, half
+  quadrupeds, half chains of 2-32 links (8 seeded models per length), in one
+  persistent launch of 20 env steps; a seeded sample of 64 envs bitwise
+  equals runs of their own, and every env stays finite."""
+  rng = np.random.default_rng(0)
+  quad = S.quadruped(0)
+  pool = {L: [S.chain(L, 1000 * L + j) for j in range(8)] for L in range(2, 33)}
+  n = 32768
+  lengths = rng.integers(2, 33, n // 2)
+  systems = [quad] * (n - n // 2) + [pool[int(L)][int(rng.integers(8))] for L in lengths]
+  systems = [systems[i] for i in rng.permutation(n)]
+  hb = hetero.HeteroBatch(systems, DEV)
+  T = 20
+  q0 = [(s.init_q + rng.uniform(-0.1, 0.1, s.q_size())).astype(np.float32) for s in systems]
+  qd0 = [(0.1 * rng.standard_normal(s.qd_size())).astype(np.float32) for s in systems]
+  acts = [rng.uniform(-1, 1, (T, s.act_size())).astype(np.float32) for s in systems]
+  q, qd, ap = hb.pack(q0, "q"), hb.pack(qd0, "qd"), hb.pack(acts, "act")
+  status = hb.rollout(q, qd, ap, T, 5)
+  torch.cuda.synchronize()
+  print(f"\n{n} mixed envs: {len(hb.groups)} models, {hb.n_items} work items, "
+        f"{hb.grid} workgroups; non-finite envs {int((status.cpu().numpy() & 2).astype(bool).sum())}",
+        flush=True)
+  assert torch.isfinite(q).all() and torch.isfinite(qd).all()
+  sample = np.sort(np.random.default_rng(1).choice(n, 64, replace=False))
+  for e in sample:
+    s = systems[e]
+    rq, rqd, _, _ = _homogeneous(s, quad, q0[e], qd0[e], acts[e])
+    sq, sqd, _ = hb.env_rows(e)
+    np.testing.assert_array_equal(q[sq].cpu().numpy(), rq)
+    np.testing.assert_array_equal(qd[sqd].cpu().numpy(), rqd)
+
+
+def test_hetero_batch_graph_replays():
+  """The batch launch captured in a HIP graph and replayed three times (the
+  work counter resets itself): each replay from the same state gives the
+  eager result."""
+  rng = np.random.default_rng(5)
+  quad = S.quadruped(0)
+  systems = [quad] * 100 + [S.chain(L, L) for L in range(2, 33)]
+  hb = hetero.HeteroBatch(systems, DEV)
+  T = 3
+  q0 = hb.pack([(s.init_q + rng.uniform(-0.1, 0.1, s.q_size())).astype(np.float32)
+                for s in systems], "q")
+  qd0 = hb.pack([(0.1 * rng.standard_normal(s.qd_size())).astype(np.float32)
+                 for s in systems], "qd")
+  ap = hb.pack([rng.uniform(-1, 1, (T, s.act_size())).astype(np.float32) for s in systems],
+               "act")
+  q, qd = q0.clone(), qd0.clone()
+  hb.launch_rollout(q, qd, ap, T, 5)
+  torch.cuda.synchronize()
+  ref_q, ref_qd = q.clone(), qd.clone()
+  side = torch.cuda.Stream(device=DEV)
+  qg, qdg = q0.clone(), qd0.clone()
+  graph = torch.cuda.CUDAGraph()
+  with torch.cuda.graph(graph, stream=side):
+    hb.launch_rollout(qg, qdg, ap, T, 5)
+  for _ in range(3):
+    qg.copy_(q0)
+    qdg.copy_(qd0)
     torch.cuda.synchronize()
-    np.testing.assert_array_equal(hb.gather(q, e).cpu().numpy(), st.q[0].cpu().numpy())
-    np.testing.assert_array_equal(hb.gather(qd, e).cpu().numpy(), st.qd[0].cpu().numpy())
+    graph.replay()
+    torch.cuda.synchronize()
+    assert torch.equal(qg, ref_q) and torch.equal(qdg, ref_qd)

@@ -96,3 +96,40 @@ def test_autotune_sets_a_candidate():
   finally:
     autotune.set_launch(m, autotune.STEP, 0)
     autotune.set_launch(m, autotune.ADJOINT, 0)
+
+
+def test_lds_tiles_agree():
+  """The quadruped's step kernels compiled for 4, 6 (default) and 8 fast
+  contact slots (the LDS tile the autotuner picks among): 64 envs with the
+  torso low enough that 2-10 candidates touch the floor, 4 env steps.  An
+  env takes the slot-per-candidate path exactly when its active contacts
+  exceed the tile (status bit), and every tile's states agree with the
+  default tile's within the band."""
+  from brax_amd.generalized import autotune
+  sys = S.quadruped(0)
+  assert autotune.tiles(sys) == (6, 4, 8)
+  rng = np.random.default_rng(41)
+  n = 64
+  q0 = sys.init_q + rng.uniform(-0.1, 0.1, (n, 15))
+  q0[:, 2] = rng.uniform(0.05, 0.3, n)
+  qd0 = 0.1 * rng.standard_normal((n, 14))
+  acts = rng.uniform(-1, 1, (4, n, 8))
+  t = lambda x: torch.tensor(np.asarray(x, np.float32), device="cuda")  # noqa: E731
+  m = pipeline.device_model(sys)
+  out = {}
+  for k in (6, 4, 8):
+    autotune.set_launch(m, autotune.TILE, k)
+    st = pipeline.init(sys, t(q0), t(qd0))
+    st, _ = pipeline.rollout(sys, st, t(acts), 5)
+    torch.cuda.synchronize()
+    out[k] = (st.q.cpu().numpy(), st.qd.cpu().numpy(), st.status.cpu().numpy())
+  autotune.set_launch(m, autotune.TILE, 0)
+  slow = {k: int(((v[2] & 1) > 0).sum()) for k, v in out.items()}
+  print(f"\nenvs on the slot-per-candidate path per tile {slow}", flush=True)
+  assert slow[4] >= slow[6] >= slow[8] and slow[4] > slow[8]
+  for k in (4, 8):
+    for a, b in zip(out[k][:2], out[6][:2]):
+      r = np.abs(a.astype(np.float64) - b) / (1e-5 + 1e-4 * np.abs(b))
+      eq = int(np.all(a == b, axis=1).sum())
+      print(f"tile {k} vs 6: max ratio {r.max():.3g}, envs bitwise equal {eq}/{n}", flush=True)
+      assert r.max() <= 1.0

@@ -83,3 +83,53 @@ def test_xml_servo_pendulum_vs_oracle(actuator):
   torch.cuda.synchronize()
   assert _ratio(st.q.cpu(), q_ref).max() <= 1.0
   assert _ratio(st.qd.cpu(), qd_ref).max() <= 1.0
+
+
+def test_reference_humanoid_xml():
+  """The reference's own humanoid (brax/envs/assets/humanoid.xml as
+  systems.humanoid_xml, pinned to load_mjcf of the file on the CPU): 11
+  links, stacked hinges (link types f2131312121), 17 limit rows, foot-sphere
+  contacts, 8 solver iterations, 20 Newton-Schulz iterations in the
+  reference (the exact inverse here).  64 envs from the humanoid env's reset
+  distribution (brax/envs/humanoid.py:210-224: q = init_q + U(-0.01, 0.01),
+  qd = U(-0.01, 0.01)) with U(-0.4, 0.4) actions, env steps of 5 substeps
+  of 0.003 s, 20 env steps free-running against the float64 oracle, the
+  float32 oracle beside it: float32 solver decisions of this model flip on
+  a few envs from the first env step (4 of 64 for both).  After one env
+  step the GPU leaves the band on at most the float32 oracle's count plus
+  one in sixteen, sorted env by env its deviations stay within 4x the
+  float32 oracle's (floored at the band) and the median env is within half
+  the band; free-running after that, where the envs that drift differ from
+  side to side, its count outside the band stays within the float32
+  oracle's plus one in eight and its median within 3x."""
+  from oracle.cstep import CStep
+  sys = S.humanoid_xml()
+  rng = np.random.default_rng(31)
+  n, T = 64, 20
+  q0 = (sys.init_q + rng.uniform(-0.01, 0.01, (n, sys.q_size()))).astype(np.float32)
+  qd0 = rng.uniform(-0.01, 0.01, (n, sys.qd_size())).astype(np.float32)
+  acts = rng.uniform(-0.4, 0.4, (T, n, sys.act_size())).astype(np.float32)
+  st = pipeline.init(sys, torch.tensor(q0, device=DEV), torch.tensor(qd0, device=DEV))
+  _, (qt, qdt) = pipeline.rollout(sys, st, torch.tensor(acts, device=DEV), 5, trajectory=True)
+  torch.cuda.synchronize()
+  qt, qdt = qt.double().cpu().numpy(), qdt.double().cpu().numpy()
+  c = CStep(sys)
+  q64, qd64 = q0.astype(np.float64), qd0.astype(np.float64)
+  q32, qd32 = q0.copy(), qd0.copy()
+  for t in range(T):
+    q64, qd64 = c.rollout(q64, qd64, acts[t:t + 1].astype(np.float64), 5, dtype=np.float64)
+    q32, qd32 = c.rollout(q32, qd32, acts[t:t + 1], 5, dtype=np.float32)
+    r = np.maximum(_ratio(qt[t], q64).max(1), _ratio(qdt[t], qd64).max(1))
+    r32 = np.maximum(_ratio(q32, q64).max(1), _ratio(qd32, qd64).max(1))
+    if t in (0, T - 1):
+      print(f"\nhumanoid.xml after {t + 1} env steps: outside the band gpu {(r > 1).sum()}/{n}, "
+            f"float32 oracle {(r32 > 1).sum()}/{n}; median ratio gpu {np.median(r):.3g}",
+            flush=True)
+    assert np.isfinite(r).all()
+    if t == 0:
+      assert np.all(np.sort(r) <= 4.0 * np.maximum(np.sort(r32), 1.0))
+      assert (r > 1).sum() <= (r32 > 1).sum() + n // 16
+      assert np.median(r) <= 0.5
+    else:  # free-running: the envs that drift differ from side to side
+      assert (r > 1).sum() <= (r32 > 1).sum() + n // 8, t
+      assert np.median(r) <= 3.0 * max(np.median(r32), 0.05), t

@@ -187,3 +187,27 @@ def test_reference_actuator_fixtures():
   t = _load("test_data/triple_pendulum_motor.xml")
   assert t.act_size() == 3 and t.link_types == "111"
   np.testing.assert_allclose(t.actuator.ctrl_range, np.tile([-100.0, 100.0], (3, 1)))
+
+
+def test_humanoid_xml_builder_equals_loaded_file():
+  """systems.humanoid_xml() (the transcription the GPU tests step, where the
+  reference tree is absent) compiles to exactly the System that load_mjcf
+  makes of brax/envs/assets/humanoid.xml: every array bitwise equal."""
+  import dataclasses
+
+  built = S.humanoid_xml()
+  loaded = _load("envs/assets/humanoid.xml")
+
+  def diff(x, y, path=""):
+    if dataclasses.is_dataclass(x):
+      return sum((diff(getattr(x, f.name), getattr(y, f.name), f"{path}.{f.name}")
+                  for f in dataclasses.fields(x)), [])
+    if isinstance(x, (np.ndarray, list, tuple)):
+      xa, ya = np.asarray(x), np.asarray(y)
+      same = xa.shape == ya.shape and np.array_equal(xa, ya, equal_nan=xa.dtype.kind == "f")
+      return [] if same else [path]
+    return [] if (x == y or path == ".name") else [path]
+
+  assert diff(built, loaded) == []
+  assert built.link_types == "f2131312121" and built.qd_size() == 23
+  assert built.num_contacts() == 2 and built.num_limit_rows() == 17
