@@ -69,6 +69,19 @@ class Config:
   seed: int = 0
 
 
+def horizon_loss(env: Env, policy: torch.nn.Module, state, horizon: int):
+  """Minus the mean reward of `horizon` env steps of the policy from state
+  (the loss of brax/training/agents/apg/train.py:176-198); returns (loss,
+  final state, observations)."""
+  rewards, observations = [], []
+  st = state
+  for _ in range(horizon):
+    observations.append(st.obs.detach())
+    st = env.step(st, policy(st.obs))
+    rewards.append(st.reward)
+  return -torch.stack(rewards).mean(), st, observations
+
+
 def train(env: Env, cfg: Config = Config(), policy: Optional[Policy] = None,
           progress: Optional[Callable[[int, Dict[str, float]], None]] = None):
   """Returns (policy, per-update metrics)."""
@@ -83,13 +96,7 @@ def train(env: Env, cfg: Config = Config(), policy: Optional[Policy] = None,
   history: List[Dict[str, float]] = []
   for it in range(cfg.num_updates):
     opt.zero_grad(set_to_none=True)
-    rewards, observations = [], []
-    st = state
-    for _ in range(cfg.horizon_length):
-      observations.append(st.obs.detach())
-      st = env.step(st, policy(st.obs))
-      rewards.append(st.reward)
-    loss = -torch.stack(rewards).mean()
+    loss, st, observations = horizon_loss(env, policy, state, cfg.horizon_length)
     loss.backward()
     params = [p for p in policy.parameters() if p.grad is not None]
     g_norm = torch.sqrt(sum((p.grad * p.grad).sum() for p in params))
@@ -101,7 +108,7 @@ def train(env: Env, cfg: Config = Config(), policy: Optional[Policy] = None,
     opt.step()
     policy.update_normalizer(torch.stack(observations))
     state = _detach(st)
-    m = {"loss": float(loss), "grad_norm": float(g_norm),
+    m = {"loss": float(loss.detach()), "grad_norm": float(g_norm),
          "params_norm": float(torch.sqrt(sum((p * p).sum() for p in policy.parameters())))}
     history.append(m)
     if progress:

@@ -7,6 +7,7 @@ import pytest
 import torch
 
 from brax_amd.envs import Ant, AutoResetWrapper, EpisodeWrapper
+from oracle_envs import OracleAnt
 
 pytestmark = pytest.mark.gpu
 
@@ -45,3 +46,61 @@ def test_ant_wrapped_episodes_restart():
   assert bool((st.done > 0).all())
   np.testing.assert_array_equal(st.pipeline_state.q.cpu().numpy(), first_q.cpu().numpy())
   assert torch.isfinite(st.info["episode_metrics"]["sum_reward"]).all()
+
+
+def _band(x, ref, rtol=1e-4, atol=1e-5):
+  x, ref = np.asarray(x, np.float64), np.asarray(ref, np.float64)
+  return np.abs(x - ref) / (atol + rtol * np.abs(ref))
+
+
+def test_ant_env_matches_oracle_env_20_steps():
+  """The Ant env on the GPU against the same env driven by the float64
+  oracle, 20 env steps of seeded U(-1, 1) actions, 64 envs.  Each step is
+  taken by the oracle env from the GPU env's own previous state (teacher
+  forcing: free-running float32 trajectories of this contact-rich task
+  drift from float64 within a few steps, see test_gpu_parity.py), and the
+  float32 oracle env takes the same step for scale: with random gear-150
+  actions the ant's stiff joint limits and foot contacts make float32
+  solver decisions flip on a fraction of env steps.  Observations (q
+  without x, y; qd) in units of the band 1e-4 relative / 1e-5 absolute,
+  rewards in units of 1e-3 + 1e-4 |r| (the forward velocity divides a
+  position difference by dt = 0.05): the GPU leaves the band on at most
+  twice the float32 oracle's env steps plus 2%, and sorted env step by env
+  step its deviations stay within 4x the float32 oracle's (floored at the
+  band); done flags equal."""
+  env, ref, ref32 = Ant(), OracleAnt(), OracleAnt(torch.float32)
+  st = env.reset(torch.Generator().manual_seed(7), 64)
+  so = ref.reset(torch.Generator().manual_seed(7), 64)
+  np.testing.assert_allclose(st.obs.cpu().numpy(), so.obs.numpy(), rtol=1e-6, atol=1e-6)
+  acts = torch.rand(20, 64, env.action_size, generator=torch.Generator().manual_seed(8)) * 2 - 1
+  r_gpu, r_32, w_gpu, w_32 = [], [], [], []
+  for t in range(20):
+    prev = st
+    st = env.step(st, acts[t].cuda())
+    torch.cuda.synchronize()
+    ps = prev.pipeline_state
+    outs = []
+    for e in (ref, ref32):
+      sp = so.replace(pipeline_state=e.pipeline_init(ps.q.cpu(), ps.qd.cpu()),
+                      obs=prev.obs.cpu().to(e.dtype))
+      outs.append(e.step(sp, acts[t]))
+    so, so32 = outs
+    ob64 = so.obs.numpy()
+    r_gpu.append(_band(st.obs.cpu().numpy(), ob64).max(1))
+    r_32.append(_band(so32.obs.numpy(), ob64).max(1))
+    rw = so.reward.numpy()
+    tol = 1e-3 + 1e-4 * np.abs(rw)
+    w_gpu.append(np.abs(st.reward.cpu().numpy() - rw) / tol)
+    w_32.append(np.abs(so32.reward.double().numpy() - rw) / tol)
+    np.testing.assert_array_equal(st.done.cpu().numpy(), so.done.numpy())
+  r_gpu, r_32 = np.concatenate(r_gpu), np.concatenate(r_32)
+  w_gpu, w_32 = np.concatenate(w_gpu), np.concatenate(w_32)
+  total = r_gpu.size
+  print(f"\nAnt env vs float64 oracle env, 20 steps x 64 envs, teacher-forced: obs outside "
+        f"the band gpu {(r_gpu > 1).sum()}/{total}, float32 oracle {(r_32 > 1).sum()}/{total}; "
+        f"median obs ratio gpu {np.median(r_gpu):.3g} float32 {np.median(r_32):.3g}; reward "
+        f"outside gpu {(w_gpu > 1).sum()} float32 {(w_32 > 1).sum()}", flush=True)
+  assert (r_gpu > 1).sum() <= 2 * (r_32 > 1).sum() + 0.02 * total
+  assert (w_gpu > 1).sum() <= 2 * (w_32 > 1).sum() + 0.02 * total
+  assert np.all(np.sort(r_gpu) <= 4.0 * np.maximum(np.sort(r_32), 1.0))
+  assert np.all(np.sort(w_gpu) <= 4.0 * np.maximum(np.sort(w_32), 1.0))
