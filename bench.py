@@ -23,9 +23,10 @@ env steps, the apg workload below) with its own roofline and CPU baseline
 Other workloads (one JSON line each, same keys):
   humanoid  16384 synthetic humanoids (23 dofs, hinge triples), 1000 env
             steps of 5 substeps (dt 0.003), actions U(-0.4, 0.4);
-  hetero    32768 envs in one HIP graph: 50% hinge chains of 2-32 links
-            with per-env model tables (drawn from 8 seeded chain models per
-            length), 50% quadrupeds; 500 env steps;
+  hetero    32768 envs in one persistent kernel launch (captured in a HIP
+            graph): 50% hinge chains of 2-32 links with per-env model tables
+            (drawn from 8 seeded chain models per length), 50% quadrupeds;
+            500 env steps;
   apg       differentiable rollout: 4096 quadrupeds, horizon 128, loss
             -sum_t v_torso,x + 1e-3 sum |act|^2, gradient w.r.t. every action
             and the initial qd by the adjoint with a checkpoint every 16 env
@@ -177,10 +178,13 @@ def forward_bench(args, name):
   acts.uniform_(-act_range, act_range, generator=g)
   q_init = torch.tensor(q0, device=dev)
   qd_init = torch.tensor(qd0, device=dev)
-  epw = 64
+  epw, kc = 64, None
   if not args.no_tune:
     from brax_amd.generalized import autotune
     epw = autotune.tune_step(sys_, q_init, qd_init, acts[:2], N_FRAMES)
+    # the tile trades occupancy against slow-path substeps, which grow as
+    # the envs settle onto the floor: time it over the timed steps (up to 20)
+    kc = autotune.tune_tile(sys_, q_init, qd_init, acts[:min(K, 20)], N_FRAMES)
   q = q_init.clone()
   qd = qd_init.clone()
   status = torch.zeros(n, dtype=torch.int32, device=dev)
@@ -273,6 +277,7 @@ def forward_bench(args, name):
                              "step, one HIP-graph rollout launch",
                  "envs": n, "env_steps": K, "substeps_per_env_step": N_FRAMES,
                  "qd_size": nv, "envs_per_workgroup": epw,
+                 "fast_contact_slots": lib.bgx_model_get_launch(m.handle, 2),
                  "parallelism": "none (1 GPU)"},
       "roofline": {"bound": "valu", "achieved": achieved,
                    "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
@@ -457,7 +462,8 @@ def apg_bench(args):
 
 
 def hetero_bench(args):
-  """Mixed batch: half chains (2-32 links, per-env tables), half quadrupeds."""
+  """Mixed batch: half chains (2-32 links, per-env tables), half quadrupeds,
+  in one persistent kernel launch."""
   import torch
   from brax_amd import systems
   from brax_amd.generalized import hetero
@@ -474,37 +480,26 @@ def hetero_bench(args):
   envs = [quad] * (n - n_chain) + [pool[int(L)][int(rng.integers(8))] for L in lengths]
   envs = [envs[i] for i in rng.permutation(n)]
   hb = hetero.HeteroBatch(envs, dev)
-  q0, qd0, acts = [], [], []
-  for g in hb.groups:
-    s0 = envs[g.envs[0]]
-    q0.append(torch.tensor((s0.init_q + rng.uniform(-0.1, 0.1, (g.n, s0.q_size())))
-                           .astype(np.float32), device=dev))
-    qd0.append(torch.tensor((0.1 * rng.standard_normal((g.n, s0.qd_size())))
-                            .astype(np.float32), device=dev))
-    a = torch.empty(max(K, W), g.n, s0.act_size(), device=dev)
-    a.uniform_(-1.0, 1.0)
-    acts.append(a)
-  q = [x.clone() for x in q0]
-  qd = [x.clone() for x in qd0]
+  q0 = hb.pack([(s.init_q + rng.uniform(-0.1, 0.1, s.q_size())).astype(np.float32)
+                for s in envs], "q")
+  qd0 = hb.pack([(0.1 * rng.standard_normal(s.qd_size())).astype(np.float32) for s in envs],
+                "qd")
+  acts = torch.empty(max(K, W), hb.act_floats, device=dev)
+  g = torch.Generator(device=dev)
+  g.manual_seed(0)
+  acts.uniform_(-1.0, 1.0, generator=g)
+  q, qd = q0.clone(), qd0.clone()
+  status = torch.zeros(n, dtype=torch.int32, device=dev)
   side = torch.cuda.Stream(device=dev)
-  epw = None
-  if not args.no_tune:
-    epw = hb.autotune(q0, qd0, acts, N_FRAMES)
-
-  def reset():
-    for a, b in zip(q, q0):
-      a.copy_(b)
-    for a, b in zip(qd, qd0):
-      a.copy_(b)
-
   with torch.cuda.stream(side):
-    hb.launch_rollout(q, qd, acts, W, N_FRAMES)
+    hb.launch_rollout(q, qd, acts, W, N_FRAMES)  # warmup
   torch.cuda.synchronize()
   graph = torch.cuda.CUDAGraph()
   with torch.cuda.graph(graph, stream=side):
-    hb.launch_rollout(q, qd, acts, K, N_FRAMES)
+    hb.launch_rollout(q, qd, acts, K, N_FRAMES, status)
   torch.cuda.synchronize()
-  reset()
+  q.copy_(q0)
+  qd.copy_(qd0)
   torch.cuda.synchronize()
   ev0 = torch.cuda.Event(enable_timing=True)
   ev1 = torch.cuda.Event(enable_timing=True)
@@ -516,15 +511,14 @@ def hetero_bench(args):
   torch.cuda.synchronize()
   wall = time.perf_counter() - t0
   kernel_ms = ev0.elapsed_time(ev1)
-  nonfinite_envs = sum(int((~(torch.isfinite(a).all(1) & torch.isfinite(b).all(1))).sum())
-                       for a, b in zip(q, qd))
-  finite = nonfinite_envs == 0
-  fl = 0.0
-  for g in hb.groups:
-    s0 = envs[g.envs[0]]
-    fl += g.n * flop_per_substep(s0.num_links(), s0.qd_size(), s0.num_contacts(),
-                                 s0.num_limit_rows(), len(s0.geom_type),
-                                 s0.solver_iterations, 9.9 if s0 is quad else 0.0)
+  nonfinite_envs = int(((status & 2) != 0).sum())
+  n_quad = n - n_chain
+  r_a = 9.9
+  fl = n_quad * flop_per_substep(quad.num_links(), quad.qd_size(), quad.num_contacts(),
+                                 quad.num_limit_rows(), len(quad.geom_type),
+                                 quad.solver_iterations, r_a)
+  for L in lengths:
+    fl += flop_per_substep(int(L), int(L), 0, 0, int(L), 4, 0.0)
   achieved = fl * K * N_FRAMES / (kernel_ms * 1e-3) / 1e12
   return {
       "metric": f"env-steps/s @{n} mixed envs (50% chains of 2-32 links, 50% quadrupeds)",
@@ -539,19 +533,17 @@ def hetero_bench(args):
       "vs_baseline": None,
       "dtype": "fp32",
       "data": "synthetic (seeded chain and quadruped states, U(-1,1) actions in HBM)",
-      "config": {"workload": "heterogeneous batch: one HIP graph of per-topology "
-                             f"rollout kernels on {len(hb.streams())} streams",
-                 "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "4"), "envs": n,
-                 "env_steps": K, "groups": len(hb.groups),
-                 "envs_per_workgroup": None if epw is None else
-                 [[envs[g.envs[0]].name, g.n, e] for g, e in zip(hb.groups, epw)],
+      "config": {"workload": "heterogeneous batch: one persistent kernel launch (bgx_batch_rollout) "
+                             "over a device work table, captured in a HIP graph",
+                 "envs": n, "env_steps": K, "models": len(hb.groups),
+                 "work_items": hb.n_items, "workgroups": hb.grid,
                  "parallelism": "none (1 GPU)"},
       "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                    "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
                    "traffic": None, "kernel_ms": kernel_ms,
-                   "note": "quadruped rows at the 65536-env mean 9.9"},
+                   "note": "op-count model: quadruped rows at R_a 9.9, chains without rows"},
       "cpu_baseline": None,
-      "finite": finite,
+      "finite": nonfinite_envs == 0,
       "nonfinite_envs": nonfinite_envs,
   }
 

@@ -33,8 +33,13 @@ ACT_STRIDE = 8  # ctrl_lo ctrl_hi force_lo force_hi gain gear bias_q bias_qd
 CAND_STRIDE = 36  # gpos[3] gquat[4] radius halflen sign mu imp[9] tw ppos[3] pn[3] t1[3] t2[3]
 IMP_SIZE = 9  # dmin dmax 1/width mid power 1/mid^(p-1) 1/(1-mid)^(p-1) b k
 
-KIND_SPHERE = 0
-KIND_CAPSULE = 1
+KIND_SPHERE = 0  # plane - sphere
+KIND_CAPSULE = 1  # plane - one capsule end
+KIND_SS = 2  # sphere - sphere
+KIND_SC = 3  # sphere - capsule
+KIND_CC = 4  # capsule - capsule
+_PAIR_CAND_KIND = {S.PAIR_SPHERE_SPHERE: KIND_SS, S.PAIR_SPHERE_CAPSULE: KIND_SC,
+                   S.PAIR_CAPSULE_CAPSULE: KIND_CC}
 
 # fast-path contact-slot capacity per topology (other candidates take the
 # scratch path); sized from the active-contact distribution of the workload
@@ -116,23 +121,39 @@ class Topology:
     else:
       self.lim_d = [int(v) for v in sys.qd_idx("123")]
       self.lim_q = [int(v) for v in sys.q_idx("123")]
-    self.cand_link, self.cand_kind, self.cand_src = [], [], []
+    # contact candidates: a plane candidate is one contact of a plane pair
+    # (the plane on the world, the sphere or capsule on link cand_link); a
+    # pair candidate is a sphere / capsule pair whose rows act on the
+    # velocity of cand_link minus that of cand_linka (-1: the world).  A
+    # pair whose geom 2 is a world geom puts geom 1's link in cand_link and
+    # sets cand_flip (the rows' directions change sign).
+    self.cand_link, self.cand_linka, self.cand_flip = [], [], []
+    self.cand_kind, self.cand_src = [], []
     for p in sys.contact_pairs():
       la = int(sys.geom_bodyid[p.geom1]) - 1
       lb = int(sys.geom_bodyid[p.geom2]) - 1
-      if la != -1 or lb < 0:
-        raise NotImplementedError("planes must belong to the world body")
-      kinds = ([(KIND_SPHERE, 0.0)] if p.kind == S.PAIR_PLANE_SPHERE
-               else [(KIND_CAPSULE, 1.0), (KIND_CAPSULE, -1.0)])
-      for kind, sign in kinds:
-        self.cand_link.append(lb)
+      if p.kind in _PAIR_CAND_KIND:
+        link, linka, flip = (la, -1, 1) if lb < 0 else (lb, la, 0)
+        cands = [(_PAIR_CAND_KIND[p.kind], 0.0)]
+      else:
+        if la != -1 or lb < 0:
+          raise NotImplementedError("planes must belong to the world body")
+        link, linka, flip = lb, -1, 0
+        cands = ([(KIND_SPHERE, 0.0)] if p.kind == S.PAIR_PLANE_SPHERE
+                 else [(KIND_CAPSULE, 1.0), (KIND_CAPSULE, -1.0)])
+      for kind, sign in cands:
+        self.cand_link.append(link)
+        self.cand_linka.append(linka)
+        self.cand_flip.append(flip)
         self.cand_kind.append(kind)
         self.cand_src.append((p, sign))
     self.ncand = len(self.cand_link)
-    # every contact plane is the z-up floor: frames and distances specialize
-    self.plane_z = all(
+    self.pairs = any(k >= KIND_SS for k in self.cand_kind)
+    # every contact plane is the z-up floor and there are no pair
+    # candidates: frames and distances specialize
+    self.plane_z = (not self.pairs) and (all(
         np.allclose(S.quat_to_mat(sys.geom_quat[p.geom1])[:, 2], [0, 0, 1], atol=0)
-        for p, _ in self.cand_src) if self.cand_src else True
+        for p, _ in self.cand_src) if self.cand_src else True)
     # per dof: 1 for a pure rotation axis (motion.vel == 0)
     self.dof_hinge = [int(not np.any(sys.dof.motion.vel[d])) for d in range(self.nv)]
     self.slots = max(1, min(slots, self.ncand)) if self.ncand else 1
@@ -165,7 +186,8 @@ class Topology:
     key = repr((self.link_types, self.parents, self.act_q, self.act_d,
                 self.lim_d, self.cand_link, self.cand_kind, self.slots,
                 self.plane_z, self.dof_hinge) + ((True,) if self.per_env else ()) +
-               ((tuple(self.tiles),) if any(self.tiles) else ()))
+               ((tuple(self.tiles),) if any(self.tiles) else ()) +
+               ((tuple(self.cand_linka), tuple(self.cand_flip)) if self.pairs else ()))
     return hashlib.sha1(key.encode()).hexdigest()[:16]
 
   def cpp_struct(self) -> str:
@@ -180,7 +202,7 @@ class Topology:
           f"KC = {self.slots}, PLANE_Z = {int(self.plane_z)}, "
           f"PER_ENV = {int(self.per_env)}, LM_LDS = {int(self.lm_lds)}, "
           f"CHAIN = {int(self.chain)}, BATCH = {int(self.batch)}, "
-          f"TILE1 = {self.tiles[0]}, TILE2 = {self.tiles[1]};\n")
+          f"TILE1 = {self.tiles[0]}, TILE2 = {self.tiles[1]}, PAIRS = {int(self.pairs)};\n")
     s += (f"  static constexpr int OFF_LINK = {self.off_link}, OFF_DOF = "
           f"{self.off_dof}, OFF_ACT = {self.off_act}, OFF_CAND = "
           f"{self.off_cand}, SIZE = {self.size};\n")
@@ -192,6 +214,9 @@ class Topology:
     s += arr("ACT_Q", self.act_q) + arr("ACT_D", self.act_d)
     s += arr("LIM_D", self.lim_d) + arr("LIM_Q", self.lim_q)
     s += arr("CAND_LINK", self.cand_link) + arr("CAND_KIND", self.cand_kind)
+    s += (arr("CAND_LINKA", self.cand_linka) if self.cand_linka else
+          "  static constexpr int CAND_LINKA[1] = {-1};\n")
+    s += arr("CAND_FLIP", self.cand_flip)
     s += arr("DOF_HINGE", self.dof_hinge)
     s += f'  static constexpr const char* SIG = "{self.signature()}";\n'
     s += "};\n"
@@ -265,6 +290,25 @@ def pack_model(sys: S.System, topo: Topology = None) -> np.ndarray:
     p, sign = topo.cand_src[c]
     o = topo.off_cand + c * CAND_STRIDE
     g2, g1 = p.geom2, p.geom1
+    if topo.cand_kind[c] >= KIND_SS:
+      # geom 2, then geom 1 in the plane's fields: local pose (the world
+      # pose for a world geom), radius, half length
+      m[o:o + 3] = sys.geom_pos[g2]
+      m[o + 3:o + 7] = sys.geom_quat[g2]
+      m[o + 7] = sys.geom_size[g2][0]
+      m[o + 8] = sys.geom_size[g2][1]
+      m[o + 10] = p.friction[0]
+      m[o + 11:o + 11 + IMP_SIZE] = impedance_record(p.solref, p.solimp)
+      la = int(sys.geom_bodyid[g1]) - 1
+      lb = int(sys.geom_bodyid[g2]) - 1
+      # constraint.py:168: invweight[link_a] (link_a > -1) + invweight[link_b];
+      # a world geom 2 (link_b = -1) indexes the last link, as jax does
+      m[o + 20] = sys.link.invweight[la] * (la > -1) + sys.link.invweight[lb]
+      m[o + 21:o + 24] = sys.geom_pos[g1]
+      m[o + 24:o + 28] = sys.geom_quat[g1]
+      m[o + 28] = sys.geom_size[g1][0]
+      m[o + 29] = sys.geom_size[g1][1]
+      continue
     m[o:o + 3] = sys.geom_pos[g2]
     m[o + 3:o + 7] = sys.geom_quat[g2]
     m[o + 7] = sys.geom_size[g2][0]
@@ -306,6 +350,8 @@ def registered_systems() -> List[Tuple[S.System, bool]]:
       (S.pendulum(3), False),
       (S.humanoid(0), False),
       (S.humanoid_xml(), False),
+      (S.two_spheres(), False),
+      (S.pile(), False),
   ]
   out += [(S.chain(n, 0), True) for n in CHAIN_LINKS]
   return out

@@ -43,9 +43,13 @@ struct AdjRec : TapeLayout<T> {
   using TL = TapeLayout<T>;
   static constexpr int QFB = TL::SIZE;               // [NV]  cotangent of qf_smooth
   static constexpr int MB = QFB + TL::NV;            // [NMB] cotangent of M (symmetrized)
-  static constexpr int SLB = MB + TL::NMB;           // [KS][12] cotangents of each slot's
-                                                     //   d0, d1, d2 and lever arm r
-  static constexpr int CAB = SLB + 12 * TL::KS;      // [3 NV] cotangent of cdof (angular)
+  // per slot: geometry in the reverse sweep's LDS (link, r, d0, d1, d2, diag;
+  // two-link slots also linka, ra) and cotangents (d0, d1, d2, r; also ra)
+  static constexpr int SLW = T::PAIRS ? 18 : 14;
+  static constexpr int SBW = T::PAIRS ? 15 : 12;
+  static constexpr int SLB = MB + TL::NMB;           // [KS][SBW] cotangents of each slot's
+                                                     //   d0, d1, d2, lever arm r (and ra)
+  static constexpr int CAB = SLB + SBW * TL::KS;     // [3 NV] cotangent of cdof (angular)
   static constexpr int CVB = CAB + 3 * TL::NV;       // [3 NV] cotangent of cdof (linear)
   static constexpr int DB = CVB + 3 * TL::NV;        // [NR]  cotangent of d
   static constexpr int AB = DB + TL::NR;             // [NR]  cotangent of aref
@@ -66,10 +70,10 @@ struct RevLds {
   static constexpr int NR = R::NR, KS = R::KS, NV = T::NV, L = T::L;
   static constexpr int XB = 0, YB = XB + NR, BB = YB + NR, DB = BB + NR, WB = DB + NR,
                        GB = WB + NR, RB = GB + NR, TR = RB + NR, TR2 = TR + NR;
-  static constexpr int SL = TR2 + NR;       // [KS][14]
-  static constexpr int SHARED = SL + 14 * KS;
-  static constexpr int SB = 0;              // [KS][12]
-  static constexpr int CAB = SB + 12 * KS;  // [3 NV]
+  static constexpr int SL = TR2 + NR;          // [KS][SLW]
+  static constexpr int SHARED = SL + R::SLW * KS;
+  static constexpr int SB = 0;                 // [KS][SBW]
+  static constexpr int CAB = SB + R::SBW * KS;  // [3 NV]
   static constexpr int CVB = CAB + 3 * NV;  // [3 NV]
   static constexpr int LV = CVB + 3 * NV;   // [L][6] link velocity (ang, vel)
   static constexpr int LF = LV + 6 * L;     // [L][6] link force (torque, force)
@@ -173,10 +177,11 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
     any_lim = any_lim || lside[i] != 0.f;
   }
   const bool has_rows = n_c > 0 || any_lim;
+  constexpr int SW = R::SLW, BW = R::SBW;
   // slot geometry into LDS, slot cotangents cleared
 #pragma unroll 1
   for (int s = 0; s < n_c; ++s) {
-    float g[14];
+    float g[SW];
     g[0] = rec[R::SLINK + s];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -186,15 +191,20 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
       g[10 + k] = rec[R::SD2 + 3 * s + k];
     }
     g[13] = rec[R::SDIAG + s];
+    if constexpr (T::PAIRS) {
+      g[14] = rec[R::SLINKA + s];
 #pragma unroll
-    for (int k = 0; k < 14; ++k) sl[14 * s + k] = g[k];
+      for (int k = 0; k < 3; ++k) g[15 + k] = rec[R::SRA + 3 * s + k];
+    }
 #pragma unroll
-    for (int k = 0; k < 12; ++k) sb[12 * s + k] = 0.f;
+    for (int k = 0; k < SW; ++k) sl[SW * s + k] = g[k];
+#pragma unroll
+    for (int k = 0; k < BW; ++k) sb[BW * s + k] = 0.f;
   }
 #pragma unroll
   for (int k = 0; k < 3 * NV; ++k) cab[k] = 0.f, cvb[k] = 0.f;
   auto slot_v3 = [&](int s, int o) {
-    return v3(sl[14 * s + o], sl[14 * s + o + 1], sl[14 * s + o + 2]);
+    return v3(sl[SW * s + o], sl[SW * s + o + 1], sl[SW * s + o + 2]);
   };
   // f(r) for every active row r
   auto for_rows = [&](auto&& f) {
@@ -254,10 +264,32 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
       }
     }
   };
-  auto slot_vel = [&](int s, V3& w, V3& v) {
-    const float* p = &lv[6 * tape_int(sl[14 * s], 0, L - 1)];
+  auto slot_link = [&](int s) { return tape_int(sl[SW * s], 0, L - 1); };
+  // second link of a two-link slot (-1: none)
+  auto slot_linka = [&](int s) {
+    if constexpr (T::PAIRS) return tape_int(sl[SW * s + 14], -1, L - 1);
+    else return -1;
+  };
+  auto link_vel_of = [&](int l, V3& w, V3& v) {
+    const float* p = &lv[6 * l];
     w = v3(p[0], p[AL], p[2 * AL]);
     v = v3(p[3 * AL], p[4 * AL], p[5 * AL]);
+  };
+  // velocity of slot s's contact point under the link velocities in lv
+  // (on its link, minus on its second link); w, wa: the two links' angular
+  // velocities (wa = 0 without a second link)
+  auto slot_pvel = [&](int s, V3& w, V3& wa) {
+    V3 v;
+    link_vel_of(slot_link(s), w, v);
+    V3 vp = v + cross(w, slot_v3(s, 1));
+    wa = v3zero();
+    const int la = slot_linka(s);
+    if (la >= 0) {
+      V3 va;
+      link_vel_of(la, wa, va);
+      vp = vp - (va + cross(wa, slot_v3(s, 15)));
+    }
+    return vp;
   };
 
   // J z on the active rows
@@ -265,9 +297,8 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
     store_link_vel(z);
 #pragma unroll 2
     for (int s = 0; s < n_c; ++s) {
-      V3 a, v;
-      slot_vel(s, a, v);
-      const V3 vp = v + cross(a, slot_v3(s, 1));
+      V3 w, wa;
+      const V3 vp = slot_pvel(s, w, wa);
       const float e0 = dot(slot_v3(s, 4), vp);
       const float e1 = dot(slot_v3(s, 7), vp);
       const float e2 = dot(slot_v3(s, 10), vp);
@@ -291,7 +322,9 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
     for (int s = 0; s < n_c; ++s) {
       float a0, a1, a2;
       const V3 w = slot_force(x, s, a0, a1, a2);
-      add_link_force(tape_int(sl[14 * s], 0, L - 1), cross(slot_v3(s, 1), w), w);
+      add_link_force(slot_link(s), cross(slot_v3(s, 1), w), w);
+      const int la = slot_linka(s);
+      if (la >= 0) add_link_force(la, cross(w, slot_v3(s, 15)), v3zero() - w);
     }
     V3 fa[L], fv[L];
     load_link_force(fa, fv);
@@ -299,7 +332,7 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
 #pragma unroll
     for (int i = 0; i < T::NLIM; ++i) tau[T::LIM_D[i]] += lside[i] * x[NRC + i];
   };
-  auto rdiag = [&](int r) { return r < NRC ? sl[14 * (r >> 2) + 13] : ldiag[r - NRC]; };
+  auto rdiag = [&](int r) { return r < NRC ? sl[SW * (r >> 2) + 13] : ldiag[r - NRC]; };
   // out = A x, ux = M^-1 J^T x
   auto A_apply = [&](const SV& x, const SV& out, float* ux) {
     JT_apply(x, ux);
@@ -364,16 +397,23 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
       float a0, a1, a2;
       const V3 f = slot_force(a, s, a0, a1, a2);
       const V3 r = slot_v3(s, 1);
-      V3 w, v;
-      slot_vel(s, w, v);
-      const V3 vp = v + cross(w, r);
+      V3 w, wa;
+      const V3 vp = slot_pvel(s, w, wa);
       const V3 rf = cross(f, w);
-      float* g = &sb[12 * s];
+      float* g = &sb[BW * s];
       g[0] += a0 * vp.x, g[AL] += a0 * vp.y, g[2 * AL] += a0 * vp.z;
       g[3 * AL] += a1 * vp.x, g[4 * AL] += a1 * vp.y, g[5 * AL] += a1 * vp.z;
       g[6 * AL] += a2 * vp.x, g[7 * AL] += a2 * vp.y, g[8 * AL] += a2 * vp.z;
       g[9 * AL] += rf.x, g[10 * AL] += rf.y, g[11 * AL] += rf.z;
-      add_link_force(tape_int(sl[14 * s], 0, L - 1), cross(r, f), f);
+      add_link_force(slot_link(s), cross(r, f), f);
+      const int la = slot_linka(s);
+      if (la >= 0) {
+        // the second link: velocity -(va + wa x ra), force -f at lever ra
+        const V3 ra = slot_v3(s, 15);
+        const V3 rfa = cross(wa, f);
+        g[12 * AL] += rfa.x, g[13 * AL] += rfa.y, g[14 * AL] += rfa.z;
+        add_link_force(la, cross(f, ra), v3zero() - f);
+      }
     }
     V3 fa[L], fv[L];
     load_link_force(fa, fv);
@@ -506,7 +546,7 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
 #pragma unroll 1
     for (int s = 0; s < n_c; ++s)
 #pragma unroll
-      for (int k = 0; k < 12; ++k) sb[12 * s + k] += pair_other(sb[12 * s + k]);
+      for (int k = 0; k < BW; ++k) sb[BW * s + k] += pair_other(sb[BW * s + k]);
   }
   // ---- actuators: qf[da] += clip(gain clip(act) + bias, force) gear
 #pragma unroll
@@ -532,11 +572,11 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
   });
 #pragma unroll 1
   for (int s = 0; s < n_c; ++s) {
-    float g[12];
+    float g[BW];
 #pragma unroll
-    for (int k = 0; k < 12; ++k) g[k] = sb[12 * s + k];
+    for (int k = 0; k < BW; ++k) g[k] = sb[BW * s + k];
 #pragma unroll
-    for (int k = 0; k < 12; ++k) rec[R::SLB + 12 * s + k] = g[k];
+    for (int k = 0; k < BW; ++k) rec[R::SLB + BW * s + k] = g[k];
   }
 #pragma unroll
   for (int k = 0; k < 3 * NV; ++k) rec[R::CAB + k] = cab[k], rec[R::CVB + k] = cvb[k];
@@ -567,7 +607,8 @@ BGX_D float adjoint_b_q(const float* __restrict__ M, const float* q_in, const fl
   V3T<S> xp[L];
   Q4T<S> xr[L];
   forward_kinematics<T, false, S>(M, q, qd, xp, xr, nullptr, nullptr);
-  const V3T<S> rc = root_com<T>(M, xp, xr);
+  V3T<S> rc[L];
+  root_coms<T>(M, xp, xr, rc);
   V3T<S> cc[L], ca[NV], cv[NV];
   Sym3T<S> ci[L];
   com_inertia<T>(M, xp, xr, rc, cc, ci);
@@ -597,22 +638,34 @@ BGX_D float adjoint_b_q(const float* __restrict__ M, const float* q_in, const fl
   for (int c = 0; c < T::NCAND; ++c) {
     const int s = tape_int(rec[R::CSLOT + c], -1, R::KS - 1);
     if (s >= 0) {
-      const int l = T::CAND_LINK[c];
+      const int l = T::CAND_LINK[c], la = T::CAND_LINKA[c];
       const float* cd = cand_rec<T>(M, c);
-      V3T<S> ctr, axis, pos, t1, t2;
-      candidate_center<T>(M, c, xp, xr, ctr, axis);
-      const S dist = candidate_dist<T>(M, c, ctr);
-      contact_frame<T>(M, c, ctr, axis, dist, tape_int(rec[R::CFB + c], 0, 1), pos, t1, t2);
-      const V3T<S> n = lift3<S>(PZ ? v3(0.f, 0.f, 1.f) : ldv3(cd + CD_PN));
+      const int fb = tape_int(rec[R::CFB + c], 0, 1);
+      V3T<S> ctr, axis, pos, n, t1, t2;
+      S dist;
+      if (T::PAIRS && T::CAND_KIND[c] >= KIND_SS) {
+        pair_contact<T>(M, c, xp, xr, fb, dist, pos, n, t1, t2);
+        if (T::CAND_FLIP[c]) n = n * -1.f, t1 = t1 * -1.f, t2 = t2 * -1.f;
+      } else {
+        candidate_center<T>(M, c, xp, xr, ctr, axis);
+        dist = candidate_dist<T>(M, c, ctr);
+        contact_frame<T>(M, c, ctr, axis, dist, fb, pos, t1, t2);
+        n = lift3<S>(PZ ? v3(0.f, 0.f, 1.f) : ldv3(cd + CD_PN));
+      }
       const float mu = cd[CD_MU];
-      const V3T<S> r = pos - rc;
+      const V3T<S> r = pos - rc[l];
+      const V3T<S> ra = la >= 0 ? pos - rc[la >= 0 ? la : 0] : r;
       const float* im = cd + CD_IMP;
       const S imp = impedance(im, dist);
       const float tw = cd[CD_TW];
       const S diag = (tw + mu * mu * tw) * (2.f * mu * mu * imp_ratio(imp));
       const S kt = im[IM_K] * imp * dist;
       const float bq = im[IM_B];
-      const V3T<S> vq = cdv[l] + cross(cda[l], r);
+      V3T<S> vq = cdv[l] + cross(cda[l], r);
+      if (la >= 0) {
+        const int lc = la >= 0 ? la : 0;
+        vq = vq - (cdv[lc] + cross(cda[lc], ra));
+      }
       const S e0 = dot(n, vq), e1 = mu * dot(t1, vq), e2 = mu * dot(t2, vq);
       const int r0 = 4 * s;
       phi += rec[R::AB + r0] * -(bq * (e0.d + e1.d) + kt.d);
@@ -623,12 +676,14 @@ BGX_D float adjoint_b_q(const float* __restrict__ M, const float* q_in, const fl
               (rec[R::DB + r0 + 2] + rec[R::DB + r0 + 3])) * diag.d;
       // J's contact rows through their ingredients (d0 = n, d1 = mu t1,
       // d2 = mu t2, r); the cdof part is contracted once below
-      const int g = R::SLB + 12 * s;
+      const int g = R::SLB + R::SBW * s;
       const V3T<S> d1 = t1 * mu, d2 = t2 * mu;
       if constexpr (!PZ) phi += rec[g] * n.x.d + rec[g + 1] * n.y.d + rec[g + 2] * n.z.d;
       phi += rec[g + 3] * d1.x.d + rec[g + 4] * d1.y.d + rec[g + 5] * d1.z.d;
       phi += rec[g + 6] * d2.x.d + rec[g + 7] * d2.y.d + rec[g + 8] * d2.z.d;
       phi += rec[g + 9] * r.x.d + rec[g + 10] * r.y.d + rec[g + 11] * r.z.d;
+      if constexpr (T::PAIRS)
+        if (la >= 0) phi += rec[g + 12] * ra.x.d + rec[g + 13] * ra.y.d + rec[g + 14] * ra.z.d;
     }
   }
 #pragma unroll
@@ -699,7 +754,8 @@ BGX_D float adjoint_b_qd(const float* __restrict__ M, const float* q_in, const f
   V3 xp[L];
   Q4 xr[L];
   forward_kinematics<T, false, float>(M, q_in, qd_in, xp, xr, nullptr, nullptr);
-  const V3 rc = root_com<T>(M, xp, xr);
+  V3 rc[L];
+  root_coms<T>(M, xp, xr, rc);
   S q[NQ], qd[NV];
 #pragma unroll
   for (int k = 0; k < NQ; ++k) q[k] = lift<S>(q_in[k]);
@@ -730,17 +786,28 @@ BGX_D float adjoint_b_qd(const float* __restrict__ M, const float* q_in, const f
   for (int c = 0; c < T::NCAND; ++c) {
     const int s = tape_int(rec[R::CSLOT + c], -1, R::KS - 1);
     if (s >= 0) {
-      const int l = T::CAND_LINK[c];
+      const int l = T::CAND_LINK[c], la = T::CAND_LINKA[c];
       const float* cd = cand_rec<T>(M, c);
-      V3 ctr, axis, pos, t1, t2;
-      candidate_center<T>(M, c, xp, xr, ctr, axis);
-      const float dist = candidate_dist<T>(M, c, ctr);
-      contact_frame<T>(M, c, ctr, axis, dist, tape_int(rec[R::CFB + c], 0, 1), pos, t1, t2);
-      const V3 n = PZ ? v3(0.f, 0.f, 1.f) : ldv3(cd + CD_PN);
+      const int fb = tape_int(rec[R::CFB + c], 0, 1);
+      V3 ctr, axis, pos, n, t1, t2;
+      float dist;
+      if (T::PAIRS && T::CAND_KIND[c] >= KIND_SS) {
+        pair_contact<T>(M, c, xp, xr, fb, dist, pos, n, t1, t2);
+        if (T::CAND_FLIP[c]) n = n * -1.f, t1 = t1 * -1.f, t2 = t2 * -1.f;
+      } else {
+        candidate_center<T>(M, c, xp, xr, ctr, axis);
+        dist = candidate_dist<T>(M, c, ctr);
+        contact_frame<T>(M, c, ctr, axis, dist, fb, pos, t1, t2);
+        n = PZ ? v3(0.f, 0.f, 1.f) : ldv3(cd + CD_PN);
+      }
       const float mu = cd[CD_MU];
-      const V3T<S> r = lift3<S>(pos - rc);
+      const V3T<S> r = lift3<S>(pos - rc[l]);
       const float bq = cd[CD_IMP + IM_B];
-      const V3T<S> vq = cdv[l] + cross(cda[l], r);
+      V3T<S> vq = cdv[l] + cross(cda[l], r);
+      if (la >= 0) {
+        const int lc = la >= 0 ? la : 0;
+        vq = vq - (cdv[lc] + cross(cda[lc], lift3<S>(pos - rc[lc])));
+      }
       const float e0 = dot(lift3<S>(n), vq).d, e1 = mu * dot(lift3<S>(t1), vq).d;
       const float e2 = mu * dot(lift3<S>(t2), vq).d;
       const int r0 = 4 * s;

@@ -93,7 +93,8 @@ __global__ __launch_bounds__(WAVE) void k_init(Launch a) {
 }
 
 // every contact candidate of the state (the reference's debug contact
-// output, brax/generalized/pipeline.py:58-59): MJX plane collisions at q
+// output, brax/generalized/pipeline.py:58-59): MJX collisions at q, frames
+// in the reference's orientation (normal from geom 1 to geom 2)
 template <class T>
 __global__ __launch_bounds__(WAVE) void k_contacts(Launch a) {
   const int env = blockIdx.x * blockDim.x + threadIdx.x;
@@ -109,11 +110,16 @@ __global__ __launch_bounds__(WAVE) void k_contacts(Launch a) {
   forward_kinematics<T, false, float>(M, q, qd, xp, xr, nullptr, nullptr);
 #pragma unroll
   for (int c = 0; c < T::NCAND; ++c) {
-    V3 ctr, axis, pos, t1, t2;
-    candidate_center<T>(M, c, xp, xr, ctr, axis);
-    const float dist = candidate_dist<T>(M, c, ctr);
-    contact_frame<T>(M, c, ctr, axis, dist, -1, pos, t1, t2);
-    const V3 n = T::PLANE_Z ? v3(0.f, 0.f, 1.f) : ldv3(cand_rec<T>(M, c) + CD_PN);
+    V3 ctr, axis, pos, n, t1, t2;
+    float dist;
+    if (T::PAIRS && T::CAND_KIND[c] >= KIND_SS) {
+      pair_contact<T>(M, c, xp, xr, -1, dist, pos, n, t1, t2);
+    } else {
+      candidate_center<T>(M, c, xp, xr, ctr, axis);
+      dist = candidate_dist<T>(M, c, ctr);
+      contact_frame<T>(M, c, ctr, axis, dist, -1, pos, t1, t2);
+      n = T::PLANE_Z ? v3(0.f, 0.f, 1.f) : ldv3(cand_rec<T>(M, c) + CD_PN);
+    }
     const long long o = (long long)env * T::NCAND + c;
     a.c_dist[o] = dist;
     a.c_pos[3 * o] = pos.x, a.c_pos[3 * o + 1] = pos.y, a.c_pos[3 * o + 2] = pos.z;

@@ -35,13 +35,20 @@ constexpr int ACT_STRIDE = 8;
 constexpr int AC_CTRL_LO = 0, AC_CTRL_HI = 1, AC_FORCE_LO = 2, AC_FORCE_HI = 3,
               AC_GAIN = 4, AC_GEAR = 5, AC_BIAS_Q = 6, AC_BIAS_QD = 7;
 
-// per contact candidate (plane vs sphere, or plane vs one capsule end)
+// per contact candidate: plane vs sphere, plane vs one capsule end, or one
+// sphere / capsule pair.  Geom 2's pose in its link's frame (the world pose
+// for a world geom), radius and half length; for a pair, geom 1's alike in
+// the space a plane candidate gives its plane
 constexpr int CAND_STRIDE = 36;
 constexpr int CD_GPOS = 0, CD_GQUAT = 3, CD_RADIUS = 7, CD_HALFLEN = 8,
               CD_SIGN = 9, CD_MU = 10, CD_IMP = 11 /* [IMP_SIZE] */,
               CD_TW = 20, CD_PPOS = 21, CD_PN = 24, CD_T1 = 27, CD_T2 = 30;
+constexpr int CD_G1POS = 21, CD_G1QUAT = 24, CD_R1 = 28, CD_HL1 = 29;
 
-constexpr int KIND_SPHERE = 0;
-constexpr int KIND_CAPSULE = 1;
+constexpr int KIND_SPHERE = 0;   // plane - sphere
+constexpr int KIND_CAPSULE = 1;  // plane - one capsule end
+constexpr int KIND_SS = 2;       // sphere - sphere
+constexpr int KIND_SC = 3;       // sphere (geom 1) - capsule (geom 2)
+constexpr int KIND_CC = 4;       // capsule - capsule
 
 }  // namespace bgx

@@ -25,6 +25,8 @@
 //    M^-1 by the LTL solve, J z as contact-point velocities).
 #pragma once
 
+#include <type_traits>
+
 #include "bgx_layout.h"
 #include "bgx_math.h"
 
@@ -91,7 +93,7 @@ BGX_D V3 dof_vel(const float* __restrict__ M, int d) {
 template <class T>
 constexpr bool hosts_contact(int l) {
   for (int c = 0; c < T::NCAND; ++c)
-    if (T::CAND_LINK[c] == l) return true;
+    if (T::CAND_LINK[c] == l || T::CAND_LINKA[c] == l) return true;
   return false;
 }
 
@@ -385,35 +387,45 @@ BGX_D void project_forces(const Tree<T>& t, const V3* fa, const V3* fv, float* t
 // dynamics pieces, generic in the scalar (float for the step, Dual for the
 // adjoint's tangent pass)
 
-// root COM of the tree (dynamics.py:37-43)
+// root COM of each link's tree (dynamics.py:37-43: the mass-weighted link
+// COMs of every link with the same root), into rc[l]
 template <class T, class S>
-BGX_D V3T<S> root_com(const float* __restrict__ M, const V3T<S>* xp, const Q4T<S>* xr) {
+BGX_D void root_coms(const float* __restrict__ M, const V3T<S>* xp, const Q4T<S>* xr,
+                     V3T<S>* rc) {
   const S zero = lift<S>(0.f);
-  V3T<S> sm{zero, zero, zero};
-  float ms = 0.f;
 #pragma unroll
-  for (int l = 0; l < T::L; ++l) {
-    const float* lk = link_rec<T>(M, l);
-    const float m = lk[LK_MASS];
-    sm = sm + (xp[l] + rotate(lift3<S>(ldv3(lk + LK_IPOS)), xr[l])) * m;
-    ms = ms + m;
+  for (int r = 0; r < T::L; ++r) {
+    if (T::PARENT[r] >= 0) continue;
+    V3T<S> sm{zero, zero, zero};
+    float ms = 0.f;
+#pragma unroll
+    for (int l = 0; l < T::L; ++l) {
+      if (T::ROOT[l] != r) continue;
+      const float* lk = link_rec<T>(M, l);
+      const float m = lk[LK_MASS];
+      sm = sm + (xp[l] + rotate(lift3<S>(ldv3(lk + LK_IPOS)), xr[l])) * m;
+      ms = ms + m;
+    }
+    const float ims = rcp(ms);
+    const V3T<S> c{sm.x * ims, sm.y * ims, sm.z * ims};
+#pragma unroll
+    for (int l = 0; l < T::L; ++l)
+      if (T::ROOT[l] == r) rc[l] = c;
   }
-  const float ims = rcp(ms);
-  return V3T<S>{sm.x * ims, sm.y * ims, sm.z * ims};
 }
 
-// cinr (dynamics.py:44-55): per link, the mass-weighted COM offset from the
-// root COM cc = m (xi - rc) and the rotational inertia about the root COM ci
+// cinr (dynamics.py:44-55): per link, the mass-weighted COM offset from its
+// tree's root COM cc = m (xi - rc) and the rotational inertia about it ci
 template <class T, class S>
 BGX_D void com_inertia(const float* __restrict__ M, const V3T<S>* xp, const Q4T<S>* xr,
-                       V3T<S> rc, V3T<S>* cc, Sym3T<S>* ci) {
+                       const V3T<S>* rc, V3T<S>* cc, Sym3T<S>* ci) {
 #pragma unroll
   for (int l = 0; l < T::L; ++l) {
     const float* lk = link_rec<T>(M, l);
     const V3T<S> xip = xp[l] + rotate(lift3<S>(ldv3(lk + LK_IPOS)), xr[l]);
     const Q4T<S> xir = qmul(xr[l], lift4<S>(ldq4(lk + LK_IROT)));
     const float m = lk[LK_MASS];
-    const V3T<S> p = xip - rc;
+    const V3T<S> p = xip - rc[l];
     S R[3][3];
     quat_to_3x3(xir, R);
     const float* i0 = lk + LK_I;
@@ -436,10 +448,11 @@ BGX_D void com_inertia(const float* __restrict__ M, const V3T<S>* xp, const Q4T<
   }
 }
 
-// cdof: dof motions about the root COM in world axes (dynamics.py:56-89)
+// cdof: dof motions about their tree's root COM in world axes
+// (dynamics.py:56-89)
 template <class T, class S>
 BGX_D void dof_motions(const float* __restrict__ M, const S* q, const V3T<S>* xp,
-                       const Q4T<S>* xr, V3T<S> rc, V3T<S>* ca, V3T<S>* cv) {
+                       const Q4T<S>* xr, const V3T<S>* rc, V3T<S>* ca, V3T<S>* cv) {
   const S zero = lift<S>(0.f);
 #pragma unroll
   for (int l = 0; l < T::L; ++l) {
@@ -473,7 +486,7 @@ BGX_D void dof_motions(const float* __restrict__ M, const S* q, const V3T<S>* xp
         }
         a = rotate(a, jr);  // dynamics.py:86-89
         ca[d] = a;
-        cv[d] = v - cross(rc - jp, a);
+        cv[d] = v - cross(rc[l] - jp, a);
       }
     }
   }
@@ -685,6 +698,110 @@ BGX_D bool contact_frame(const float* __restrict__ M, int c, V3T<S> ctr, V3T<S> 
 }
 
 // ---------------------------------------------------------------------------
+// sphere and capsule pairs (MJX collision_primitive sphere_sphere,
+// sphere_capsule, capsule_capsule and the segment helpers of its math
+// module).  Geom 1 has the lower type; the normal points from geom 1 to
+// geom 2.  A pair candidate's links: CAND_LINK holds a body link; when
+// geom 2 is a world geom (CAND_FLIP) that link carries geom 1, else geom 2,
+// and CAND_LINKA carries geom 1 (-1: the world).
+
+template <class S>
+BGX_D S clamp_s(S x, S lo, S hi) {
+  return val(x) < val(lo) ? lo : (val(x) > val(hi) ? hi : x);
+}
+
+// the point of segment a-b closest to p (projection regularized by 1e-6)
+template <class S>
+BGX_D V3T<S> seg_point(V3T<S> a, V3T<S> b, V3T<S> p) {
+  const V3T<S> ab = b - a;
+  const S t = dot(p - a, ab) * rcp(dot(ab, ab) + 1e-6f);
+  return a + ab * clamp_s(t, lift<S>(0.f), lift<S>(1.f));
+}
+
+// closest points of segments a0-a1 and b0-b1: the lines' closest points
+// clipped to the segments, then the nearer of the two back-projections
+template <class S>
+BGX_D void seg_seg(V3T<S> a0, V3T<S> a1, V3T<S> b0, V3T<S> b1, V3T<S>& pa, V3T<S>& pb) {
+  S la, lb;
+  const V3T<S> da = vnormalize(a1 - a0, &la), db = vnormalize(b1 - b0, &lb);
+  const S ha = la * 0.5f, hb = lb * 0.5f;
+  const V3T<S> am = a0 + da * ha, bm = b0 + db * hb;
+  const V3T<S> tr = am - bm;
+  const S dd = dot(da, db), dat = dot(da, tr), dbt = dot(db, tr);
+  const S ta0 = (dd * dbt - dat) * rcp((1.f - dd * dd) + 1e-6f);
+  const S tb0 = dbt + ta0 * dd;
+  const V3T<S> ba = am + da * clamp_s(ta0, lift<S>(0.f) - ha, ha);
+  const V3T<S> bb = bm + db * clamp_s(tb0, lift<S>(0.f) - hb, hb);
+  const V3T<S> na = seg_point(a0, a1, bb), nb = seg_point(b0, b1, ba);
+  const V3T<S> ea = bb - na, eb = ba - nb;
+  const bool first = val(dot(ea, ea)) < val(dot(eb, eb));
+  pa = first ? na : ba;
+  pb = first ? bb : nb;
+}
+
+// contact of spheres (p1, r1), (p2, r2): n from 1 to 2 (x for coincident
+// centres), dist = |p2 - p1| - r1 - r2, pos = p1 + n (r1 + dist / 2), frame
+// [a, b, a x b] with a = n normalized and b from orthogonals(a) (helper axis
+// y when |a.y| < 1/2, else z; fallback < 0: decide here, else 0 / 1 = y / z).
+// Returns whether z was the helper.
+template <class S>
+BGX_D bool sphere_pair(V3T<S> p1, float r1, V3T<S> p2, float r2, int fallback, S& dist,
+                       V3T<S>& pos, V3T<S>& n, V3T<S>& t1, V3T<S>& t2) {
+  S nn;
+  V3T<S> u = vnormalize(p2 - p1, &nn);
+  if (val(nn) == 0.f) u = lift3<S>(v3(1.f, 0.f, 0.f));
+  dist = nn - (r1 + r2);
+  pos = p1 + u * (r1 + dist * 0.5f);
+  n = vnormalize(u, (S*)nullptr);
+  const bool fb = fallback < 0 ? !(-0.5f < val(n.y) && val(n.y) < 0.5f) : fallback != 0;
+  const V3T<S> h = lift3<S>(fb ? v3(0.f, 0.f, 1.f) : v3(0.f, 1.f, 0.f));
+  t1 = vnormalize(h - n * dot(n, h), (S*)nullptr);
+  t2 = cross(n, t1);
+  return fb;
+}
+
+// world pose of a candidate's geom (contact.py:42-53): at (pos, quat) in
+// link l's frame, or at that world pose for l < 0; axis: its z axis
+template <class S>
+BGX_D V3T<S> geom_pos(const float* cd, int off, int l, const V3T<S>* xp, const Q4T<S>* xr) {
+  if (l < 0) return lift3<S>(ldv3(cd + off));
+  return xp[l] + rotate(lift3<S>(ldv3(cd + off)), xr[l]);
+}
+template <class S>
+BGX_D V3T<S> geom_axis(const float* cd, int off, int l, const Q4T<S>* xr) {
+  if (l < 0) return lift3<S>(quat_zaxis(ldq4(cd + off)));
+  return quat_zaxis(qmul(xr[l], lift4<S>(ldq4(cd + off))));
+}
+
+template <class T>
+constexpr int pair_link1(int c) { return T::CAND_FLIP[c] ? T::CAND_LINK[c] : T::CAND_LINKA[c]; }
+template <class T>
+constexpr int pair_link2(int c) { return T::CAND_FLIP[c] ? -1 : T::CAND_LINK[c]; }
+
+// contact of pair candidate c in the reference's orientation (normal from
+// geom 1 to geom 2); returns the frame's helper-axis choice
+template <class T, class S>
+BGX_D bool pair_contact(const float* __restrict__ M, int c, const V3T<S>* xp, const Q4T<S>* xr,
+                        int fallback, S& dist, V3T<S>& pos, V3T<S>& n, V3T<S>& t1,
+                        V3T<S>& t2) {
+  const float* cd = cand_rec<T>(M, c);
+  const int kind = T::CAND_KIND[c], l1 = pair_link1<T>(c), l2 = pair_link2<T>(c);
+  V3T<S> p1 = geom_pos<S>(cd, CD_G1POS, l1, xp, xr);
+  V3T<S> p2 = geom_pos<S>(cd, CD_GPOS, l2, xp, xr);
+  if (kind == KIND_SC) {
+    const V3T<S> sg = geom_axis<S>(cd, CD_GQUAT, l2, xr) * cd[CD_HALFLEN];
+    p2 = seg_point(p2 - sg, p2 + sg, p1);
+  } else if (kind == KIND_CC) {
+    const V3T<S> s1 = geom_axis<S>(cd, CD_G1QUAT, l1, xr) * cd[CD_HL1];
+    const V3T<S> s2 = geom_axis<S>(cd, CD_GQUAT, l2, xr) * cd[CD_HALFLEN];
+    V3T<S> c1, c2;
+    seg_seg(p1 - s1, p1 + s1, p2 - s2, p2 + s2, c1, c2);
+    p1 = c1, p2 = c2;
+  }
+  return sphere_pair(p1, cd[CD_R1], p2, cd[CD_RADIUS], fallback, dist, pos, n, t1, t2);
+}
+
+// ---------------------------------------------------------------------------
 // position update of the semi-implicit Euler step (integrator.py:26-44)
 
 // free-joint rotation: rot (x) quat_rot_axis(w / (|w| + 1e-8), dt (|w| + 1e-8)),
@@ -739,6 +856,16 @@ struct SlotGeo<false> {
   V3 r, d0, d1, d2;
 };
 
+// a contact between two links (sphere / capsule pairs): rows act on the
+// velocity of the contact point on `link` minus that on `linka` (-1: the
+// world), with lever arms r and ra from each link's root COM
+struct SlotGeoP {
+  int link, linka;
+  V3 r, ra, d0, d1, d2;
+};
+template <class T>
+using SlotOf = std::conditional_t<T::PAIRS != 0, SlotGeoP, SlotGeo<T::PLANE_Z != 0>>;
+
 BGX_D void slot_set(SlotGeo<true>& g, int link, V3 r, V3 n, V3 d1, V3 d2) {
   (void)n, (void)d2;
   g.link = link, g.r = r, g.ax = d1.x, g.ay = d1.y;
@@ -754,6 +881,9 @@ BGX_D V3 slot_force(const SlotGeo<true>& g, float v0, float v1, float v2, float 
 BGX_D V3 slot_force(const SlotGeo<false>& g, float v0, float v1, float v2, float v3) {
   return g.d0 * ((v0 + v1) + (v2 + v3)) + g.d1 * (v0 - v1) + g.d2 * (v2 - v3);
 }
+BGX_D V3 slot_force(const SlotGeoP& g, float v0, float v1, float v2, float v3) {
+  return g.d0 * ((v0 + v1) + (v2 + v3)) + g.d1 * (v0 - v1) + g.d2 * (v2 - v3);
+}
 // (n.vp, mu t1.vp, mu t2.vp) for a contact-point velocity vp
 BGX_D void slot_dots(const SlotGeo<true>& g, V3 vp, float& e0, float& e1, float& e2) {
   e0 = vp.z;
@@ -762,6 +892,41 @@ BGX_D void slot_dots(const SlotGeo<true>& g, V3 vp, float& e0, float& e1, float&
 }
 BGX_D void slot_dots(const SlotGeo<false>& g, V3 vp, float& e0, float& e1, float& e2) {
   e0 = dot(g.d0, vp), e1 = dot(g.d1, vp), e2 = dot(g.d2, vp);
+}
+BGX_D void slot_dots(const SlotGeoP& g, V3 vp, float& e0, float& e1, float& e2) {
+  e0 = dot(g.d0, vp), e1 = dot(g.d1, vp), e2 = dot(g.d2, vp);
+}
+// velocity of slot g's contact point from the link velocities (ang, vel) at
+// f + 6 l WAVE: on link, minus on linka for a two-link slot
+template <class G>
+BGX_D V3 slot_point_vel(const G& g, const float* f) {
+  const float* a = f + g.link * 6 * WAVE;
+  V3 v = v3(a[3 * WAVE], a[4 * WAVE], a[5 * WAVE]) + cross(v3(a[0], a[WAVE], a[2 * WAVE]), g.r);
+  if constexpr (std::is_same_v<G, SlotGeoP>) {
+    if (g.linka >= 0) {
+      const float* b = f + g.linka * 6 * WAVE;
+      v = v - (v3(b[3 * WAVE], b[4 * WAVE], b[5 * WAVE]) +
+               cross(v3(b[0], b[WAVE], b[2 * WAVE]), g.ra));
+    }
+  }
+  return v;
+}
+// adds force w at the contact point of slot g to the link forces (torque,
+// force) at f + 6 l WAVE: +w on link, -w on linka
+template <class G>
+BGX_D void slot_add_force(const G& g, V3 w, float* f) {
+  const V3 ta = cross(g.r, w);
+  float* a = f + g.link * 6 * WAVE;
+  a[0] += ta.x, a[WAVE] += ta.y, a[2 * WAVE] += ta.z;
+  a[3 * WAVE] += w.x, a[4 * WAVE] += w.y, a[5 * WAVE] += w.z;
+  if constexpr (std::is_same_v<G, SlotGeoP>) {
+    if (g.linka >= 0) {
+      const V3 tb = cross(g.ra, w);
+      float* b = f + g.linka * 6 * WAVE;
+      b[0] -= tb.x, b[WAVE] -= tb.y, b[2 * WAVE] -= tb.z;
+      b[3 * WAVE] -= w.x, b[4 * WAVE] -= w.y, b[5 * WAVE] -= w.z;
+    }
+  }
 }
 
 // per-row solver vectors of the slots: x, y (FISTA iterates), g (gradient),
@@ -840,7 +1005,11 @@ struct TapeLayout {
   static constexpr int XS = RS + ADJ_MAXITER * NR;    // [ADJ_MAXITER][NR] x_{k+1}
   static constexpr int UY = XS + ADJ_MAXITER * NR;    // [ADJ_MAXITER][NV] M^-1 J^T y_k
   static constexpr int UR = UY + ADJ_MAXITER * NV;    // [ADJ_MAXITER][NV] M^-1 J^T r_k
-  static constexpr int SIZE = UR + ADJ_MAXITER * NV;
+  // two-link slots (topologies with sphere / capsule pairs)
+  static constexpr int KSP = T::PAIRS ? KS : 0;
+  static constexpr int SLINKA = UR + ADJ_MAXITER * NV;  // [KSP] second link (-1: none)
+  static constexpr int SRA = SLINKA + KSP;              // [3 KSP] its lever arm
+  static constexpr int SIZE = SRA + 3 * KSP;
 };
 
 struct NoTape {
@@ -870,6 +1039,9 @@ BGX_D void slot_dirs(const SlotGeo<true>& g, V3& d0, V3& d1, V3& d2) {
 BGX_D void slot_dirs(const SlotGeo<false>& g, V3& d0, V3& d1, V3& d2) {
   d0 = g.d0, d1 = g.d1, d2 = g.d2;
 }
+BGX_D void slot_dirs(const SlotGeoP& g, V3& d0, V3& d1, V3& d2) {
+  d0 = g.d0, d1 = g.d1, d2 = g.d2;
+}
 template <class Tp>
 BGX_D void tape_v3(const Tp& tape, int f, V3 v) {
   tape[f] = v.x, tape[f + 1] = v.y, tape[f + 2] = v.z;
@@ -882,7 +1054,7 @@ BGX_D void tape_v3(const Tp& tape, int f, V3 v) {
 
 template <class T, int KS, class Rows, class Tape = NoTape>
 BGX_D bool substep_impl(const float* __restrict__ M, float* q, float* qd,
-                        const float* act, const V3* xp, const Q4* xr, V3 rc0,
+                        const float* act, const V3* xp, const Q4* xr, const V3* rcl,
                         float* lds, Rows& rows, float* dbg, Tape tape = Tape{}) {
   using TL = TapeLayout<T>;
   constexpr int NV = T::NV, L = T::L;
@@ -892,28 +1064,48 @@ BGX_D bool substep_impl(const float* __restrict__ M, float* q, float* qd,
   if constexpr (T::LM_LDS) t.lml = lds + lds_lm_offset<T>() * WAVE;
   float* lds2 = lds + L * 6 * WAVE;  // second link-vector area (rows space)
 #pragma unroll
-  for (int l = 0; l < L; ++l) t.rc[l] = rc0;
+  for (int l = 0; l < L; ++l) t.rc[l] = rcl[l];
 
-  // ---- contacts (contact.py + MJX plane collisions) into slots
-  SlotGeo<PZ> sg[KS];
+  // ---- contacts (contact.py + MJX collisions) into slots
+  using SG = SlotOf<T>;
+  SG sg[KS];
   float sl_diag[KS], sl_bq[KS], sl_kt[KS];
   int n_c = 0;
 #pragma unroll
   for (int c = 0; c < T::NCAND; ++c) {
-    V3 ctr, axis;
-    candidate_center<T>(M, c, xp, xr, ctr, axis);
-    const float dist = candidate_dist<T>(M, c, ctr);
+    const float* cd = cand_rec<T>(M, c);
+    const bool pair = T::PAIRS && T::CAND_KIND[c] >= KIND_SS;
+    float dist;
+    V3 ctr, axis, pos, n, t1, t2;
+    bool fb = false;
+    if (pair) {
+      fb = pair_contact<T>(M, c, xp, xr, -1, dist, pos, n, t1, t2);
+    } else {
+      candidate_center<T>(M, c, xp, xr, ctr, axis);
+      dist = candidate_dist<T>(M, c, ctr);
+    }
     if constexpr (Tape::ON) tape[TL::CSLOT + c] = -1.f, tape[TL::CFB + c] = 0.f;
     if (dist < 0.f) {
       const int l = T::CAND_LINK[c];
-      const float* cd = cand_rec<T>(M, c);
-      V3 pos, t1, t2;
-      const bool fb = contact_frame<T>(M, c, ctr, axis, dist, -1, pos, t1, t2);
+      if (!pair) {
+        fb = contact_frame<T>(M, c, ctr, axis, dist, -1, pos, t1, t2);
+        n = PZ ? v3(0.f, 0.f, 1.f) : ldv3(cd + CD_PN);
+      }
       if constexpr (Tape::ON) tape[TL::CSLOT + c] = (float)n_c, tape[TL::CFB + c] = fb ? 1.f : 0.f;
-      const V3 n = PZ ? v3(0.f, 0.f, 1.f) : ldv3(cd + CD_PN);
       const float mu = cd[CD_MU];
-      SlotGeo<PZ> g;
-      slot_set(g, l, pos - t.rc[l], n, t1 * mu, t2 * mu);
+      SG g;
+      if constexpr (T::PAIRS) {
+        // rows act on (geom 2's link - geom 1's link): a world geom 2 turns
+        // them onto geom 1's link with the directions negated
+        const float sgn = T::CAND_FLIP[c] ? -1.f : 1.f;
+        const int la = T::CAND_LINKA[c];
+        g.link = l, g.linka = la;
+        g.r = pos - t.rc[l];
+        g.ra = la >= 0 ? pos - t.rc[la >= 0 ? la : 0] : v3zero();
+        g.d0 = n * sgn, g.d1 = t1 * (mu * sgn), g.d2 = t2 * (mu * sgn);
+      } else {
+        slot_set(g, l, pos - t.rc[l], n, t1 * mu, t2 * mu);
+      }
       const float* im = cd + CD_IMP;
       const float imp = impedance(im, dist);
       const float tw = cd[CD_TW];
@@ -945,6 +1137,10 @@ BGX_D bool substep_impl(const float* __restrict__ M, float* q, float* qd,
         slot_dirs(sg[s], d0, d1, d2);
         tape[TL::SLINK + s] = (float)sg[s].link;
         tape_v3(tape, TL::SR + 3 * s, sg[s].r);
+        if constexpr (T::PAIRS) {
+          tape[TL::SLINKA + s] = (float)sg[s].linka;
+          tape_v3(tape, TL::SRA + 3 * s, sg[s].ra);
+        }
         tape_v3(tape, TL::SD0 + 3 * s, d0);
         tape_v3(tape, TL::SD1 + 3 * s, d1);
         tape_v3(tape, TL::SD2 + 3 * s, d2);
@@ -955,8 +1151,8 @@ BGX_D bool substep_impl(const float* __restrict__ M, float* q, float* qd,
   // ---- dynamics.transform_com (dynamics.py:27-89): cinr, cdof
   V3 cc[L];
   Sym3 ci[L];
-  com_inertia<T>(M, xp, xr, t.rc[0], cc, ci);
-  dof_motions<T>(M, q, xp, xr, t.rc[0], t.ca, t.cv);
+  com_inertia<T>(M, xp, xr, t.rc, cc, ci);
+  dof_motions<T>(M, q, xp, xr, t.rc, t.ca, t.cv);
 
   // ---- cd: link velocities at qd (dynamics.py:90-103), also kept in lds2
   V3 cda[L], cdv[L];
@@ -1025,13 +1221,8 @@ BGX_D bool substep_impl(const float* __restrict__ M, float* q, float* qd,
 #pragma unroll
   for (int s = 0; s < KS; ++s) {
     if (s < n_c) {
-      const float* fu = lds + sg[s].link * 6 * WAVE;
-      const float* fq = lds2 + sg[s].link * 6 * WAVE;
-      const V3 r = sg[s].r;
-      const V3 vu = v3(fu[3 * WAVE], fu[4 * WAVE], fu[5 * WAVE]) +
-                    cross(v3(fu[0], fu[WAVE], fu[2 * WAVE]), r);
-      const V3 vq = v3(fq[3 * WAVE], fq[4 * WAVE], fq[5 * WAVE]) +
-                    cross(v3(fq[0], fq[WAVE], fq[2 * WAVE]), r);
+      const V3 vu = slot_point_vel(sg[s], lds);
+      const V3 vq = slot_point_vel(sg[s], lds2);
       float e0u, e1u, e2u, e0q, e1q, e2q;
       slot_dots(sg[s], vu, e0u, e1u, e2u);
       slot_dots(sg[s], vq, e0q, e1q, e2q);
@@ -1084,14 +1275,7 @@ BGX_D bool substep_impl(const float* __restrict__ M, float* q, float* qd,
       if (s < n_c) {
         const V3 w = slot_force(sg[s], in_slot(s, 0), in_slot(s, 1),
                                 in_slot(s, 2), in_slot(s, 3));
-        const V3 ta = cross(sg[s].r, w);
-        float* f = lds + sg[s].link * 6 * WAVE;
-        f[0 * WAVE] += ta.x;
-        f[1 * WAVE] += ta.y;
-        f[2 * WAVE] += ta.z;
-        f[3 * WAVE] += w.x;
-        f[4 * WAVE] += w.y;
-        f[5 * WAVE] += w.z;
+        slot_add_force(sg[s], w, lds);
       }
     }
     V3 fa[L], fv[L];
@@ -1131,11 +1315,8 @@ BGX_D bool substep_impl(const float* __restrict__ M, float* q, float* qd,
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
       if (s < n_c) {
-        const float* f = lds + sg[s].link * 6 * WAVE;
-        const V3 a = v3(f[0], f[WAVE], f[2 * WAVE]);
-        const V3 v = v3(f[3 * WAVE], f[4 * WAVE], f[5 * WAVE]);
         float e0, e1, e2;
-        slot_dots(sg[s], v + cross(a, sg[s].r), e0, e1, e2);
+        slot_dots(sg[s], slot_point_vel(sg[s], lds), e0, e1, e2);
         const float dg = sl_diag[s];
         out_slot(s, 0, e0 + e1 + dg * in_slot(s, 0));
         out_slot(s, 1, e0 - e1 + dg * in_slot(s, 1));
@@ -1330,11 +1511,11 @@ BGX_D bool substep_ks(const float* __restrict__ M, float* q, float* qd,
                       const float* act, float* lds, Rows& rows, float* dbg,
                       Tape tape = Tape{}) {
   static_assert(T::L > 0, "empty topology");
-  static_assert(T::ROOT[T::L - 1] == 0, "one kinematic tree per topology");
   V3 xp[T::L];
   Q4 xr[T::L];
   forward_kinematics<T, false, float>(M, q, qd, xp, xr, nullptr, nullptr);
-  const V3 rc = root_com<T>(M, xp, xr);  // dynamics.py:37-43
+  V3 rc[T::L];
+  root_coms<T>(M, xp, xr, rc);  // dynamics.py:37-43
   return substep_impl<T, KS>(M, q, qd, act, xp, xr, rc, lds, rows, dbg, tape);
 }
 

@@ -33,8 +33,10 @@ class Contact:
   """Contacts of a state (brax/base.py Contact; the generalized pipeline's
   debug output): every candidate of the topology, active (dist < 0) or not.
   pos (N, C, 3), frame (N, C, 3, 3) rows normal, t1, t2, dist (N, C);
-  friction (C,) the tangential coefficient, solref (C, 2), solimp (C, 5);
-  link_idx = (link of the plane, link of the geom), each (C,) (-1 = world)."""
+  friction (C,) the tangential coefficient, solref (C, 2), solimp (C, 5),
+  elasticity (C,) the mean of the two geoms'; link_idx = (link of geom 1,
+  link of geom 2), each (C,) (-1 = world), the frame's normal pointing from
+  geom 1 to geom 2."""
   pos: torch.Tensor
   frame: torch.Tensor
   dist: torch.Tensor
@@ -42,6 +44,7 @@ class Contact:
   solref: torch.Tensor
   solimp: torch.Tensor
   link_idx: tuple
+  elasticity: Optional[torch.Tensor] = None
 
 
 @dataclasses.dataclass

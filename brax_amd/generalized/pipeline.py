@@ -142,8 +142,8 @@ def _frames(m: DeviceModel, n: int, device):
 def contacts(sys: S.System, q) -> Optional[Contact]:
   """The contacts of states q (N, q_size): what the reference's pipeline
   attaches to its State with debug=True (brax/generalized/pipeline.py:58-59,
-  91-92), from the GPU's plane collisions; None for a topology without
-  contact candidates."""
+  91-92), from the GPU's collisions; None for a topology without contact
+  candidates."""
   m = device_model(sys, q.device if torch.is_tensor(q) and q.is_cuda else None)
   qb = _as_batch(q, m.nq, m.device, "q").detach()
   n = qb.shape[0]
@@ -163,10 +163,12 @@ def contacts(sys: S.System, q) -> Optional[Contact]:
   first = sys.systems[0] if isinstance(sys, SystemBatch) else sys
   la = [int(first.geom_bodyid[p.geom1]) - 1 for p in srcs]
   lb = [int(first.geom_bodyid[p.geom2]) - 1 for p in srcs]
+  el = [0.5 * (float(first.elasticity[p.geom1]) + float(first.elasticity[p.geom2])) for p in srcs]
   return Contact(pos=pos, frame=frame, dist=dist,
                  friction=t([p.friction[0] for p in srcs]),
                  solref=t([p.solref for p in srcs]), solimp=t([p.solimp for p in srcs]),
-                 link_idx=(torch.tensor(la, device=m.device), torch.tensor(lb, device=m.device)))
+                 link_idx=(torch.tensor(la, device=m.device), torch.tensor(lb, device=m.device)),
+                 elasticity=t(el))
 
 
 def _with_contacts(sys, state: State, debug: bool) -> State:

@@ -41,12 +41,20 @@ GEOM_PLANE = 0
 GEOM_SPHERE = 2
 GEOM_CAPSULE = 3
 
-# contact counts per (type1, type2) pair handled by the step (MJX narrow phase)
+# pair kinds handled by the step (MJX narrow phase), keyed by (type1, type2)
+# with type1 <= type2, and their contact counts
 PAIR_PLANE_SPHERE = 0
 PAIR_PLANE_CAPSULE = 1
+PAIR_SPHERE_SPHERE = 2
+PAIR_SPHERE_CAPSULE = 3
+PAIR_CAPSULE_CAPSULE = 4
 _PAIR_KIND = {(GEOM_PLANE, GEOM_SPHERE): PAIR_PLANE_SPHERE,
-              (GEOM_PLANE, GEOM_CAPSULE): PAIR_PLANE_CAPSULE}
-_PAIR_NCON = {PAIR_PLANE_SPHERE: 1, PAIR_PLANE_CAPSULE: 2}
+              (GEOM_PLANE, GEOM_CAPSULE): PAIR_PLANE_CAPSULE,
+              (GEOM_SPHERE, GEOM_SPHERE): PAIR_SPHERE_SPHERE,
+              (GEOM_SPHERE, GEOM_CAPSULE): PAIR_SPHERE_CAPSULE,
+              (GEOM_CAPSULE, GEOM_CAPSULE): PAIR_CAPSULE_CAPSULE}
+_PAIR_NCON = {PAIR_PLANE_SPHERE: 1, PAIR_PLANE_CAPSULE: 2, PAIR_SPHERE_SPHERE: 1,
+              PAIR_SPHERE_CAPSULE: 1, PAIR_CAPSULE_CAPSULE: 1}
 
 _MJ_MINVAL = 1e-15
 _DEFAULT_SOLREF = (0.02, 1.0)
@@ -117,8 +125,8 @@ class Opt:
 class ContactPair:
   """One collision pair after MJX's filtering and parameter mixing."""
 
-  kind: int  # PAIR_PLANE_SPHERE or PAIR_PLANE_CAPSULE
-  geom1: int  # the plane
+  kind: int  # PAIR_*
+  geom1: int  # the geom of the lower type (the plane of a plane pair)
   geom2: int
   ncon: int
   friction: np.ndarray  # (5,)
@@ -224,7 +232,9 @@ class System:
     return d
 
   def contact_pairs(self) -> List[ContactPair]:
-    """Collision pairs with MJX filtering and mixing; plane is geom1."""
+    """Collision pairs with MJX filtering and mixing: geom1 has the lower
+    geom type (the lower geom id between equal types), so the plane of a
+    plane pair is geom1 and contact normals point from geom1 to geom2."""
     body_parent = [-1] + [p + 1 for p in self.link_parents]  # MuJoCo body ids
     pairs = []
     ng = len(self.geom_type)
@@ -404,6 +414,7 @@ class _Geom:
   solimp: np.ndarray
   solmix: float
   name: str
+  elasticity: float = 0.0
 
 
 @dataclasses.dataclass
@@ -464,7 +475,7 @@ class ModelBuilder:
                mass: Optional[float] = None, contype: int = 1,
                conaffinity: int = 1, friction=(1.0, 0.005, 0.0001),
                solref=_DEFAULT_SOLREF, solimp=_DEFAULT_SOLIMP,
-               solmix: float = 1.0, name: str = ""):
+               solmix: float = 1.0, name: str = "", elasticity: float = 0.0):
     size = np.array(list(size) + [0.0] * (3 - len(size)), np.float64)
     pos = np.array(pos, np.float64)
     quat = np.array(quat, np.float64) / np.linalg.norm(quat)
@@ -478,7 +489,8 @@ class ModelBuilder:
     self.geoms.append(_Geom(body, typ, size, pos, quat, float(mass), contype,
                             conaffinity, np.array(friction, np.float64),
                             np.array(solref, np.float64),
-                            np.array(solimp, np.float64), solmix, name))
+                            np.array(solimp, np.float64), solmix, name,
+                            float(elasticity)))
 
   def add_motor(self, joint: str, gear: float, ctrlrange=None):
     self.add_actuator(joint, gear, ctrlrange=ctrlrange)
@@ -620,7 +632,9 @@ class ModelBuilder:
     if self.init_q_override is not None:
       init_q = np.array(self.init_q_override, np.float64)
 
-    g = self.geoms
+    # MuJoCo numbers geoms body by body (the world's first), in declaration
+    # order within a body; pair order and roles follow these ids
+    g = sorted(self.geoms, key=lambda x: x.body)
     sys = System(
         name=self.name,
         gravity=self.gravity.copy(),
@@ -656,7 +670,7 @@ class ModelBuilder:
         geom_solmix=np.array([x.solmix for x in g], np.float64),
         geom_contype=np.array([x.contype for x in g], np.int64),
         geom_conaffinity=np.array([x.conaffinity for x in g], np.int64),
-        elasticity=np.zeros(len(g)),
+        elasticity=np.array([x.elasticity for x in g], np.float64),
         geom_names=[x.name for x in g],
     )
     _set_invweights(sys, qpos0)
@@ -1147,6 +1161,52 @@ def chain(n_links: int, seed: int = 0) -> System:
   return b.build()
 
 
+def two_spheres() -> System:
+  """Two free bodies whose spheres overlap by 0.01 at the initial pose: body1
+  at the origin with a sphere of radius 0.2 (elasticity 0.1), body2 at the
+  origin with a sphere of radius 0.11 at (0, 0.3, 0); geom elasticity
+  default 0.2; MuJoCo defaults otherwise (density 1000, timestep 0.002,
+  100 solver iterations).  The system of the reference's sphere-sphere
+  contact test (brax/contact_test.py:45-75)."""
+  b = ModelBuilder("two_spheres", timestep=0.002, iterations=100)
+  b1 = b.add_body("body1", 0)
+  b.add_free_joint(b1, "free1")
+  b.add_geom(b1, GEOM_SPHERE, (0.2,), name="sphere1", elasticity=0.1)
+  b2 = b.add_body("body2", 0)
+  b.add_free_joint(b2, "free2")
+  b.add_geom(b2, GEOM_SPHERE, (0.11,), pos=(0.0, 0.3, 0.0), name="sphere2", elasticity=0.2)
+  return b.build()
+
+
+PILE_BODIES = ("ball1", "ball2", "rod1", "rod2")
+
+
+def pile() -> System:
+  """Four free bodies over a floor, beside a capsule fixed to the world: two
+  spheres (radii 0.1, 0.08) and two capsules (radius 0.05, half length 0.2
+  along x; radius 0.04, half length 0.15 along y), density 500.  Every geom
+  collides with every other (the world's two with the bodies'), so the
+  contacts cover each pair kind the step handles: plane-sphere,
+  plane-capsule, sphere-sphere, sphere-capsule and capsule-capsule, with a
+  world geom on either side of a pair.  Friction 0.8 (tangential), timestep
+  0.005, 6 solver iterations; the bodies start apart and settle onto each
+  other and the floor."""
+  b = ModelBuilder("pile", timestep=0.005, iterations=6)
+  kw = dict(density=500.0, friction=(0.8, 0.005, 0.0001))
+  b.add_geom(0, GEOM_PLANE, (10, 10, 0.1), name="floor", **kw)
+  b.add_geom(0, GEOM_CAPSULE, (0.06,), fromto=(-0.6, -0.25, 0.06, 0.6, -0.25, 0.06),
+             name="rail", **kw)
+  specs = [("ball1", (0.0, -0.06, 0.13), GEOM_SPHERE, (0.1,), None),
+           ("ball2", (0.22, 0.05, 0.1), GEOM_SPHERE, (0.08,), None),
+           ("rod1", (0.05, 0.02, 0.36), GEOM_CAPSULE, (0.05,), (-0.2, 0, 0, 0.2, 0, 0)),
+           ("rod2", (0.0, -0.05, 0.55), GEOM_CAPSULE, (0.04,), (0, -0.15, 0, 0, 0.15, 0))]
+  for name, pos, typ, size, fromto in specs:
+    body = b.add_body(name, 0, pos=pos)
+    b.add_free_joint(body, name)
+    b.add_geom(body, typ, size, fromto=fromto, name=name, **kw)
+  return b.build()
+
+
 def free_body(gravity=(0.0, 0.0, -9.81), radius: float = 0.1,
               mass: float = 1.0, timestep: float = 0.01,
               pos=(0.0, 0.0, 1.0)) -> System:
@@ -1361,6 +1421,25 @@ def load_mjcf(xml: str, name: Optional[str] = None) -> System:
                      bias_q=bq, bias_qd=bqd,
                      ctrlrange=tuple(crng) if limited(a, "ctrllimited", crng) and crng else None,
                      forcerange=tuple(frng) if limited(a, "forcelimited", frng) and frng else None)
+  # per-geom elasticity (mjcf.py:154-233): the custom numeric sets every
+  # geom's value (one value, or one per geom in geom-id order), then custom
+  # tuple elements override it per named geom
+  if "elasticity" in custom:
+    v = custom["elasticity"]
+    order = sorted(b.geoms, key=lambda x: x.body)
+    if len(v) not in (1, len(order)):
+      raise ValueError(f"custom elasticity needs 1 or {len(order)} values, got {len(v)}")
+    for i, g in enumerate(order):
+      g.elasticity = float(v[0] if len(v) == 1 else v[i])
+  for c in root.findall("custom"):
+    for tup in c.findall("tuple"):
+      if tup.get("name") != "elasticity":
+        continue
+      by_name = {g.name: g for g in b.geoms if g.name}
+      for el in tup.findall("element"):
+        if el.get("objtype") != "geom":
+          raise NotImplementedError("custom elasticity tuples hold geoms")
+        by_name[el.get("objname")].elasticity = float(el.get("prm"))
   b.bodies = _fuse_jointless(b)
   if "init_qpos" in custom:
     b.init_q_override = custom["init_qpos"]

@@ -7,14 +7,17 @@ not under the reference tree; they are restated from their published source:
 
 * ``pg_solve``: jaxopt.ProjectedGradient (FISTA with backtracking line search,
   ``jaxopt`` unpinned in pyproject.toml:41; call site constraint.py:229-238).
-* ``collide``: MJX plane-sphere and plane-capsule narrow phase
-  (``mujoco-mjx`` unpinned, pyproject.toml:45; call site contact.py:57).
+* ``collide``: MJX narrow phase for plane-sphere, plane-capsule,
+  sphere-sphere, sphere-capsule and capsule-capsule pairs (``mujoco-mjx``,
+  pyproject.toml:44-45; call site contact.py:57).
 
 Parity pins (tests that would fail on a plausible mistake):
   scan schedules ........ scan_test.py KATs (tests/test_oracle_kat.py)
   spatial algebra ....... closed forms (rotation orthogonality, q*q^-1)
   kinematics ............ hand-derived pendulum FK; FD of positions vs xd
-  contact geometry ...... contact_test.py:29-44 ant KAT; closed forms
+  contact geometry ...... contact_test.py:29-44 ant KAT and :67-75 sphere-sphere
+                          KAT; closed forms; segment closest points against
+                          brute force
   mass matrix ........... brute-force sum_links J^T I J (independent code)
   RNEA bias ............. FD of potential energy at qd=0; FD of momentum
   actuator .............. actuator_test.py:57-60 and :114-118 closed forms
@@ -128,6 +131,71 @@ def orthogonals(a):
   b = b - a * dot(a, b)[..., None]
   b = normalize(b)[0] * (a != 0).any(-1, keepdim=True).to(a.dtype)
   return b, cross(a, b)
+
+
+# -- MJX narrow-phase primitives for sphere and capsule pairs ----------------
+# mujoco-mjx (pyproject.toml:44-45, mujoco>=3.8.1; not under the reference
+# tree): mjx/_src/collision_primitive.py (_sphere_sphere, sphere_capsule,
+# capsule_capsule) and mjx/_src/math.py (make_frame,
+# closest_segment_point[_and_dist], closest_segment_to_segment_points),
+# restated from the published source; called through contact.py:57.
+
+
+def make_frame(a):
+  """A right-handed frame with rows a / |a|, then orthogonals of it."""
+  a = normalize(a)[0]
+  b, c = orthogonals(a)
+  return torch.stack([a, b, c], -2)
+
+
+def sphere_sphere(pos1, r1, pos2, r2):
+  """One contact of two spheres: normal from sphere 1 to sphere 2 (the x
+  axis for coincident centres), dist = |p2 - p1| - r1 - r2, pos midway
+  through the overlap."""
+  n, d = normalize(pos2 - pos1)
+  ex = torch.tensor([1.0, 0.0, 0.0], dtype=pos1.dtype).expand_as(n)
+  n = torch.where((d == 0.0)[..., None], ex, n)
+  dist = d - (r1 + r2)
+  pos = pos1 + n * (r1 + dist * 0.5)[..., None]
+  return dist, pos, make_frame(n)
+
+
+def closest_segment_point(a, b, pt):
+  """The point of segment a-b closest to pt (a 1e-6 regularized projection)."""
+  ab = b - a
+  t = dot(pt - a, ab) / (dot(ab, ab) + 1e-6)
+  return a + torch.clamp(t, 0.0, 1.0)[..., None] * ab
+
+
+def closest_segment_point_and_dist(a, b, pt):
+  c = closest_segment_point(a, b, pt)
+  return c, dot(pt - c, pt - c)
+
+
+def closest_segment_to_segment_points(a0, a1, b0, b1):
+  """Closest points of segments a0-a1 and b0-b1: the closest points of the
+  two lines clipped to the segments, then, if the other segment's point
+  projected back is nearer, that pair."""
+  dir_a, len_a = normalize(a1 - a0)
+  dir_b, len_b = normalize(b1 - b0)
+  half_a, half_b = len_a * 0.5, len_b * 0.5
+  a_mid = a0 + dir_a * half_a[..., None]
+  b_mid = b0 + dir_b * half_b[..., None]
+  trans = a_mid - b_mid
+  dd = dot(dir_a, dir_b)
+  dat = dot(dir_a, trans)
+  dbt = dot(dir_b, trans)
+  denom = 1 - dd * dd
+  ta0 = (-dat + dd * dbt) / (denom + 1e-6)
+  tb0 = dbt + ta0 * dd
+  ta = torch.minimum(torch.maximum(ta0, -half_a), half_a)
+  tb = torch.minimum(torch.maximum(tb0, -half_b), half_b)
+  best_a = a_mid + dir_a * ta[..., None]
+  best_b = b_mid + dir_b * tb[..., None]
+  new_a, d1 = closest_segment_point_and_dist(a0, a1, best_b)
+  new_b, d2 = closest_segment_point_and_dist(b0, b1, best_a)
+  c = (d1 < d2)[..., None]
+  return torch.where(c, new_a, best_a), torch.where(c, best_b, new_b)
 
 
 def inv_approximate(a, a_inv, num_iter):
@@ -598,10 +666,37 @@ class Oracle:
       return pos, mat
 
     out = {k: [] for k in ("dist", "pos", "frame", "friction", "solref",
-                           "solimp", "link_a", "link_b")}
+                           "solimp", "link_a", "link_b", "elasticity")}
+    def add(p, dist, pos, frame):
+      out["dist"].append(dist)
+      out["pos"].append(pos)
+      out["frame"].append(frame)
+      out["friction"].append(T(p.friction).expand(N, 5))
+      out["solref"].append(T(p.solref).expand(N, 2))
+      out["solimp"].append(T(p.solimp).expand(N, 5))
+      out["link_a"].append(int(s.geom_bodyid[p.geom1]) - 1)
+      out["link_b"].append(int(s.geom_bodyid[p.geom2]) - 1)
+      out["elasticity"].append(0.5 * (float(s.elasticity[p.geom1]) +
+                                      float(s.elasticity[p.geom2])))
+
     for p in self.pairs:
       ppos, pmat = geom_pose(p.geom1)
       gpos, gmat = geom_pose(p.geom2)
+      if p.kind >= 2:  # sphere and capsule pairs: one contact each
+        r1 = float(s.geom_size[p.geom1][0])
+        r2 = float(s.geom_size[p.geom2][0])
+        if p.kind == 2:  # sphere-sphere
+          add(p, *sphere_sphere(ppos, r1, gpos, r2))
+        elif p.kind == 3:  # sphere (geom1) - capsule (geom2)
+          seg = gmat[..., :, 2] * float(s.geom_size[p.geom2][1])
+          pt = closest_segment_point(gpos - seg, gpos + seg, ppos)
+          add(p, *sphere_sphere(ppos, r1, pt, r2))
+        else:  # capsule-capsule
+          s1 = pmat[..., :, 2] * float(s.geom_size[p.geom1][1])
+          s2 = gmat[..., :, 2] * float(s.geom_size[p.geom2][1])
+          pt1, pt2 = closest_segment_to_segment_points(ppos - s1, ppos + s1, gpos - s2, gpos + s2)
+          add(p, *sphere_sphere(pt1, r1, pt2, r2))
+        continue
       n = pmat[..., :, 2]  # plane normal: z column
       r = float(s.geom_size[p.geom2][0])
       if p.kind == 0:  # plane-sphere: frame from orthogonals of n
@@ -621,20 +716,13 @@ class Oracle:
       for ctr in centers:
         dist = dot(ctr - ppos, n) - r
         pos = ctr - n * (r + 0.5 * dist)[..., None]
-        out["dist"].append(dist)
-        out["pos"].append(pos)
-        out["frame"].append(frame)
-        out["friction"].append(T(p.friction).expand(N, 5))
-        out["solref"].append(T(p.solref).expand(N, 2))
-        out["solimp"].append(T(p.solimp).expand(N, 5))
-        out["link_a"].append(int(s.geom_bodyid[p.geom1]) - 1)
-        out["link_b"].append(int(s.geom_bodyid[p.geom2]) - 1)
+        add(p, dist, pos, frame)
     if not out["dist"]:
       return None
     res = {k: torch.stack(v, 1) for k, v in out.items()
-           if k not in ("link_a", "link_b")}
-    res["link_a"] = out["link_a"]
-    res["link_b"] = out["link_b"]
+           if k not in ("link_a", "link_b", "elasticity")}
+    for k in ("link_a", "link_b", "elasticity"):
+      res[k] = out[k]
     return res
 
   # -- constraint.jacobian (constraint.py:29-191) ----------------------------
