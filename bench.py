@@ -179,6 +179,9 @@ def forward_bench(args, name):
   q_init = torch.tensor(q0, device=dev)
   qd_init = torch.tensor(qd0, device=dev)
   epw, kc = 64, None
+  if args.lanes:
+    from brax_amd.generalized import autotune
+    autotune.set_launch(m, autotune.LANES, args.lanes)
   if not args.no_tune:
     from brax_amd.generalized import autotune
     epw = autotune.tune_step(sys_, q_init, qd_init, acts[:2], N_FRAMES)
@@ -277,6 +280,7 @@ def forward_bench(args, name):
                              "step, one HIP-graph rollout launch",
                  "envs": n, "env_steps": K, "substeps_per_env_step": N_FRAMES,
                  "qd_size": nv, "envs_per_workgroup": epw,
+                 "lanes_per_env": lib.bgx_model_get_launch(m.handle, 3),
                  "fast_contact_slots": lib.bgx_model_get_launch(m.handle, 2),
                  "parallelism": "none (1 GPU)"},
       "roofline": {"bound": "valu", "achieved": achieved,
@@ -556,6 +560,9 @@ def main():
   ap.add_argument("--envs", type=int, default=0)
   ap.add_argument("--workload", default="quadruped", choices=sorted(WORKLOADS))
   ap.add_argument("--ckpt", type=int, default=16)
+  ap.add_argument("--lanes", type=int, default=0,
+                  help="lanes per env of the step kernels (1: one thread per env; 0: the "
+                       "topology's default)")
   ap.add_argument("--no-graph", action="store_true")
   ap.add_argument("--no-cpu", action="store_true")
   ap.add_argument("--no-adjoint", action="store_true",

@@ -58,6 +58,12 @@ SLOT_TILES = {"quadruped": (4, 8)}
 # heterogeneous workload mixes chains with quadrupeds
 BATCH_SYSTEMS = {"quadruped"}
 
+# tree systems whose step also compiles the multi-lane tree kernel
+# (bgx_mtree.h: a lane per dof, the default for their homogeneous
+# launches): large trees whose batches leave the chip short of wavefronts
+# at one thread per env
+MLANE_SYSTEMS = {"humanoid", "humanoid_xml"}
+
 JAXOPT_TOL = 1e-3  # jaxopt ProximalGradient default tol
 LS_EPS = float(np.finfo(np.float32).eps)
 
@@ -171,6 +177,7 @@ class Topology:
     # the persistent heterogeneous-batch kernel is compiled for this tree
     # topology (its envs can share a launch with chains)
     self.batch = (not self.chain) and sys.name in BATCH_SYSTEMS
+    self.mlane = sys.name in MLANE_SYSTEMS and self._mlane_family(sys)
     # LTL factor and limit-row solver vectors in LDS instead of registers:
     # supported, but off for every registered topology (the 16384-humanoid
     # batch ran at 5.2e5 env-steps/s with it against 5.8e5 without: at one
@@ -182,12 +189,32 @@ class Topology:
     self.off_cand = self.off_act + ACT_STRIDE * max(self.nu, 1)
     self.size = self.off_cand + CAND_STRIDE * max(self.ncand, 1)
 
+  def _mlane_family(self, sys) -> bool:
+    """The multi-lane tree kernel's family (bgx_mtree.h mtree_ok): one tree
+    rooted at link 0, a free joint only there, no sphere / capsule pairs,
+    at most 32 dofs and candidates, each actuator on its own dof's
+    hinge coordinate, one actuator per dof."""
+    if self.per_env or self.pairs or self.nv > 32 or self.ncand > 32:
+      return False
+    if self.parents[0] >= 0 or any(p < 0 for p in self.parents[1:]):
+      return False
+    if any(t == 0 for t in self.ltype[1:]) or (self.ltype[0] == 0 and self.qadr[0] != 0):
+      return False
+    if len(set(self.act_d)) != len(self.act_d):
+      return False
+    for q_, d_ in zip(self.act_q, self.act_d):
+      l = self.dof_link[d_]
+      if self.ltype[l] == 0 or q_ != self.qadr[l] + (d_ - self.dadr[l]):
+        return False
+    return True
+
   def signature(self) -> str:
     key = repr((self.link_types, self.parents, self.act_q, self.act_d,
                 self.lim_d, self.cand_link, self.cand_kind, self.slots,
                 self.plane_z, self.dof_hinge) + ((True,) if self.per_env else ()) +
                ((tuple(self.tiles),) if any(self.tiles) else ()) +
-               ((tuple(self.cand_linka), tuple(self.cand_flip)) if self.pairs else ()))
+               ((tuple(self.cand_linka), tuple(self.cand_flip)) if self.pairs else ()) +
+               (("mlane",) if self.mlane else ()))
     return hashlib.sha1(key.encode()).hexdigest()[:16]
 
   def cpp_struct(self) -> str:
@@ -202,7 +229,8 @@ class Topology:
           f"KC = {self.slots}, PLANE_Z = {int(self.plane_z)}, "
           f"PER_ENV = {int(self.per_env)}, LM_LDS = {int(self.lm_lds)}, "
           f"CHAIN = {int(self.chain)}, BATCH = {int(self.batch)}, "
-          f"TILE1 = {self.tiles[0]}, TILE2 = {self.tiles[1]}, PAIRS = {int(self.pairs)};\n")
+          f"TILE1 = {self.tiles[0]}, TILE2 = {self.tiles[1]}, PAIRS = {int(self.pairs)}, "
+          f"MLANE = {int(self.mlane)};\n")
     s += (f"  static constexpr int OFF_LINK = {self.off_link}, OFF_DOF = "
           f"{self.off_dof}, OFF_ACT = {self.off_act}, OFF_CAND = "
           f"{self.off_cand}, SIZE = {self.size};\n")

@@ -81,6 +81,7 @@ struct bgx_model {
   int64_t n_tables;  // tables (one per env for per-env topologies)
   int epw[2];        // envs per workgroup: step kernels, adjoint kernels (0 = 64)
   int kc;            // fast contact slots of the step kernels (0: the default)
+  int lanes;         // lanes per env of the step kernels (0: the default)
 };
 
 extern "C" {
@@ -157,6 +158,13 @@ int bgx_model_set_launch(bgx_model* m, int kernels, int32_t envs_per_workgroup) 
     m->kc = k;
     return BGX_OK;
   }
+  if (kernels == BGX_KERNELS_LANES) {
+    const int k = envs_per_workgroup;
+    if (k != 0 && k != 1 && k != kOps[m->topo]->mlanes)
+      return fail(BGX_ERR_ARG, "lanes per env must be 0, 1 or the topology's multi-lane width");
+    m->lanes = k;
+    return BGX_OK;
+  }
   if (kernels != BGX_KERNELS_STEP && kernels != BGX_KERNELS_ADJOINT)
     return fail(BGX_ERR_ARG, "unknown kernel group");
   const int e = envs_per_workgroup;
@@ -168,6 +176,10 @@ int bgx_model_set_launch(bgx_model* m, int kernels, int32_t envs_per_workgroup) 
 
 int bgx_model_get_launch(const bgx_model* m, int kernels) {
   if (m && kernels == BGX_KERNELS_TILE) return m->kc ? m->kc : kOps[m->topo]->tiles[0];
+  if (m && kernels == BGX_KERNELS_LANES) {
+    const int w = kOps[m->topo]->mlanes;
+    return m->lanes ? m->lanes : (w ? w : 1);
+  }
   if (!m || (kernels != BGX_KERNELS_STEP && kernels != BGX_KERNELS_ADJOINT)) return -1;
   return m->epw[kernels] ? m->epw[kernels] : bgx::WAVE;
 }
@@ -184,6 +196,7 @@ static int launch(const bgx_model* m, bool init_only, bgx::Launch a, void* strea
   a.M = m->table;
   a.epw = m->epw[BGX_KERNELS_STEP];
   a.kc = m->kc;
+  a.lanes = m->lanes;
   const bgx::TopoOps* ops = kOps[m->topo];
   a.q_step = (long long)a.n * ops->dims[1], a.qd_step = (long long)a.n * ops->dims[2];
   if ((a.q_traj || a.qd_traj) && a.n_seg > 1)
@@ -423,7 +436,8 @@ int rollout_adjoint_impl(const bgx_model* m, int32_t n, int32_t n_steps, int32_t
     f.sub_q = w.sub_q, f.sub_qd = w.sub_qd;
     f.status = status, f.status_or = 1;
     f.epw = m->epw[BGX_KERNELS_ADJOINT];
-    f.kc = m->kc;  // the forward's tile: the recompute repeats its states
+    f.kc = m->kc;  // the forward's tile and kernel: the recompute repeats its states
+    f.lanes = m->lanes;
     if ((rc = check_hip(ops->rollout(f, s), "recompute launch"))) return rc;
     for (int seg = seg_hi; seg >= seg_lo; --seg) {
       const int s0 = seg * ckpt_every;

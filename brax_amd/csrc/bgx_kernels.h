@@ -17,6 +17,7 @@
 #include "bgx_adjoint.h"
 #include "bgx_batch.h"
 #include "bgx_launch.h"
+#include "bgx_mtree.h"
 #include "bgx_step.h"
 
 namespace bgx {
@@ -458,6 +459,7 @@ struct TopoOps {
   hipError_t (*contacts)(const Launch&, hipStream_t);
   hipError_t (*frames_vjp)(const FramesVjp&, hipStream_t);
   int chain;     // a hinge-chain topology (bgx_chain.h kernels)
+  int mlanes;    // lanes per env of the multi-lane tree kernel (0: not compiled)
   int tiles[3];  // fast contact slots the step kernels are compiled for (0: none)
   // the persistent batch kernel that runs this topology's envs beside
   // chains (null: the topology cannot join a heterogeneous batch), and its
@@ -484,6 +486,18 @@ struct Ops {
     return hipGetLastError();
   }
   static hipError_t rollout(const Launch& a, hipStream_t s) {
+    if constexpr (T::MLANE) {
+      static_assert(mtree_ok<T>(), "topology outside the multi-lane tree kernel's family");
+      if (a.lanes != 1) {
+        constexpr int E = MTab<T>::E;
+        hipLaunchKernelGGL(k_mtree<T>,
+                           dim3((unsigned)((a.n + E - 1) / E), a.n_seg > 1 ? (unsigned)a.n_seg : 1u),
+                           dim3(WAVE), 0, s, a);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess || !(a.x_pos || a.x_rot || a.xd_ang || a.xd_vel)) return e;
+        return init(a, s);  // frames of the final state
+      }
+    }
     if constexpr (T::TILE1 > 0) {
       if (a.kc == T::TILE1) return rollout_tile<Tile<T, T::TILE1>>(a, s);
     }
@@ -536,6 +550,7 @@ struct Ops {
               nullptr};
     if constexpr (T::NCAND > 0) t.contacts = &contacts;
     t.frames_vjp = &frames_vjp;
+    t.mlanes = T::MLANE ? MTab<T>::G : 0;
     t.tiles[0] = T::KC, t.tiles[1] = T::TILE1, t.tiles[2] = T::TILE2;
     if constexpr (T::BATCH) t.batch = &launch_batch<T>, t.batch_occupancy = &batch_blocks_per_cu<T>;
     // the adjoint is compiled for topologies with a shared model table

@@ -65,6 +65,9 @@ struct Launch {
   // a homogeneous launch of a tree topology: its fast contact slots (the
   // LDS tile; 0 = the topology's default)
   int kc;
+  // lanes per env of a homogeneous tree launch: 1 = one thread per env,
+  // 0 = the multi-lane kernel where the topology has it (bgx_mtree.h)
+  int lanes;
   // batch mode (items non-null): a persistent launch over the work table;
   // the arrays are packed and the items give each env's offsets; probe,
   // frames and substep states are not written

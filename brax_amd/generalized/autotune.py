@@ -29,6 +29,7 @@ from brax_amd.generalized import pipeline
 STEP = 0
 ADJOINT = 1
 TILE = 2  # the step kernels' LDS tile: fast contact slots per env
+LANES = 3  # the step kernels' lanes per env (1, or the multi-lane width)
 CANDIDATES: Tuple[int, ...] = (64, 32, 16, 8, 4)
 
 _CACHE: Dict[tuple, int] = {}

@@ -93,6 +93,12 @@ int bgx_model_topology(const bgx_model* model);
  * rounding across tiles (an env with more active contacts than slots takes
  * the slot-per-candidate path). */
 #define BGX_KERNELS_TILE 2
+/* kernels = BGX_KERNELS_LANES sets the step kernels' lanes per env: 1 runs
+ * one thread per env; the topology's multi-lane width (the default, 0,
+ * where the topology is compiled with the multi-lane tree kernel) runs an
+ * env on a group of lanes (a lane per dof).  The two agree within rounding;
+ * the getter returns the width in use. */
+#define BGX_KERNELS_LANES 3
 int bgx_model_set_launch(bgx_model* model, int kernels, int32_t envs_per_workgroup);
 /* the current setting (64 for 0), or -1 on bad arguments */
 int bgx_model_get_launch(const bgx_model* model, int kernels);
