@@ -173,6 +173,8 @@ struct MLane {
   int ll, lld;     // the link this lane holds (or -1), its depth
   int c;           // contact candidate (or -1)
   int li, ai;      // limit row and actuator of the dof (or -1)
+  unsigned cmask;  // the candidates on the dof's subtree (bit c)
+  unsigned pmask;  // the dofs on the candidate's root path (bit d)
   float rq[7];     // the root's free-joint coordinates (every lane)
   float qh, qd;    // this dof's hinge coordinate and velocity
 };
@@ -191,6 +193,11 @@ BGX_D MLane<T> mlane_roles(int g, int gb) {
   m.c = g < T::NCAND ? g : -1;
   m.li = m.d >= 0 ? MT::tab.lim_of_dof[dc] : -1;
   m.ai = m.d >= 0 ? MT::tab.act_of_dof[dc] : -1;
+  m.cmask = 0u, m.pmask = 0u;
+  if (m.d >= 0)
+    for (int k = 0; k < MT::tab.ncsub[dc]; ++k) m.cmask |= 1u << MT::tab.csub[dc][k];
+  if (m.c >= 0)
+    for (int k = 0; k < MT::tab.npath[m.c]; ++k) m.pmask |= 1u << MT::tab.path[m.c][k];
   return m;
 }
 
@@ -581,7 +588,7 @@ BGX_D void mtree_substep(const float* __restrict__ M, MLane<T>& s, const float* 
   // M^-1 times the per-dof vector in ZV, on this dof
   auto minv_mul = [&]() {
     float z = 0.f;
-#pragma unroll 4
+#pragma unroll
     for (int j = 0; j < NV; ++j) z += mrow[j] * lds[MT::ZV + j];
     return z;
   };
@@ -590,10 +597,9 @@ BGX_D void mtree_substep(const float* __restrict__ M, MLane<T>& s, const float* 
   // order of dynamics.py:93-103's scan)
   auto path_vel = [&](int base, V3& wa, V3& wv) {
     wa = v3zero(), wv = v3zero();
-#pragma unroll 4
-    for (int k = 0; k < MT::MAXPATH; ++k) {
-      if (c >= 0 && k < tb.npath[cc_]) {
-        const int dd = tb.path[cc_][k];
+#pragma unroll
+    for (int dd = 0; dd < NV; ++dd) {  // ancestors first: dof indices grow down the tree
+      if ((s.pmask >> dd) & 1u) {
         const float* cf = lds + MT::CDF + 6 * dd;
         const float z = lds[base + dd];
         wa = wa + ld3l(cf) * z;
@@ -691,10 +697,10 @@ BGX_D void mtree_substep(const float* __restrict__ M, MLane<T>& s, const float* 
     // candidates on its subtree (dynamics.py:174-182's sums, gathered)
     float tau = 0.f;
     if (d >= 0) {
-#pragma unroll 4
-      for (int k = 0; k < MT::MAXCSUB; ++k) {
-        if (k < tb.ncsub[d]) {
-          const float* cf = lds + MT::CF + 6 * tb.csub[d][k];
+#pragma unroll
+      for (int k = 0; k < NC; ++k) {
+        if ((s.cmask >> k) & 1u) {
+          const float* cf = lds + MT::CF + 6 * k;
           tau += dot(cv, ld3l(cf + 3)) + dot(ca, ld3l(cf));
         }
       }
@@ -738,25 +744,24 @@ BGX_D void mtree_substep(const float* __restrict__ M, MLane<T>& s, const float* 
     if (!wave_any(running)) break;
     // gradient at y: r = A y + b, g = A^T r
     float o[4], ol;
-    apply_a(Y, yl, o, ol);
+    {
+      apply_a(Y, yl, o, ol);
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (running) Rr[k] = active ? o[k] + sb[k] : 0.f;
+      if (running) rl = ol + lb;
+    }
     float fy = 0.f;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const float rv = o[k] + sb[k];
-      if (running) Rr[k] = rv;
-      fy += active ? 0.5f * rv * rv : 0.f;
-    }
-    {
-      const float rv = ol + lb;
-      if (running) rl = rv;
-      fy += has_l ? 0.5f * rv * rv : 0.f;
-    }
+    for (int k = 0; k < 4; ++k) fy += active ? 0.5f * Rr[k] * Rr[k] : 0.f;
+    fy += has_l ? 0.5f * rl * rl : 0.f;
     fy = gsum<G>(fy);
-    apply_a(Rr, rl, o, ol);
-    if (running) {
+    {
+      apply_a(Rr, rl, o, ol);
 #pragma unroll
-      for (int k = 0; k < 4; ++k) Gr[k] = o[k];
-      gl = ol;
+      for (int k = 0; k < 4; ++k)
+        if (running) Gr[k] = o[k];
+      if (running) gl = ol;
     }
     // line search: trial x+ = relu(y - eta g), residual into r
     bool cond = running;
@@ -819,10 +824,13 @@ BGX_D void mtree_substep(const float* __restrict__ M, MLane<T>& s, const float* 
         const float xk = active ? xp[k] : 0.f;
         Y[k] = active ? xk + beta * (xk - X[k]) : 0.f;
         X[k] = xk;
+        // the next y's gradient; o holds A (A x+ + b)
+
       }
       if (has_l) {
         yl = xpl + beta * (xpl - xl);
         xl = xpl;
+
       }
       eta = eta <= 1e-6f ? 1.f : eta * 2.f;
       tk = tn;

@@ -63,8 +63,8 @@ def test_humanoid_first_substep_forces_vs_oracle(humanoid):
   the band on every env.  qdd comes out of the first contact solve, where
   float32 rounding flips solver decisions on most of these envs (the
   float32 oracle leaves the band on 10 of 16): qdd in the band on every env
-  whose float32 result stays in it under one-ulp input changes, and within
-  K_EXEMPT of that float32 spread on the others."""
+  whose float32 result stays in it under twelve one-ulp input changes, and
+  within K_EXEMPT of that float32 spread on the others."""
   import oracle.generalized as G
   from test_gpu_parity import K_EXEMPT
   sys = humanoid
@@ -75,7 +75,7 @@ def test_humanoid_first_substep_forces_vs_oracle(humanoid):
   r64 = o64.step(o64.init(q0.astype(np.float64), qd0.astype(np.float64)), a0.astype(np.float64))
   rng = np.random.default_rng(0)
   spread = np.zeros(q0.shape[0])
-  for i in range(5):
+  for i in range(13):
     ulp = (lambda x: x) if i == 0 else \
         (lambda x: x * (1 + rng.choice([-1.0, 1.0], x.shape) * 2.0 ** -23))
     r32 = o32.step(o32.init(ulp(q0).astype(np.float32), ulp(qd0).astype(np.float32)),
