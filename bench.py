@@ -96,12 +96,18 @@ def workload(name, n, seed=0):
   return sys_, q0, qd0, rng
 
 
-def measured_traffic(name, n, steps):
+def profile_path(kernel, name, n):
+  """The committed rocprofv3 summary of `kernel` on workload `name` at `n` envs."""
+  return os.path.join("profiles", f"{kernel}_{name}_{n}.md")
+
+
+def measured_traffic(kernel, name, n, steps):
   """HBM bytes of one rollout launch of `steps` env steps of `n` envs, from
   the committed rocprofv3 --pmc summary of that same command (FETCH_SIZE +
   WRITE_SIZE of the rollout kernel, calibrated with a copy kernel), or None
-  when no profile of this workload, batch and step count is committed."""
-  path = os.path.join(ROOT, "profiles", f"k_rollout_{name}_{n}.md")
+  when no profile of this kernel, workload, batch and step count is
+  committed."""
+  path = os.path.join(ROOT, profile_path(kernel, name, n))
   if not os.path.exists(path):
     return None
   for line in open(path):
@@ -182,7 +188,13 @@ def forward_bench(args, name):
   if args.lanes:
     from brax_amd.generalized import autotune
     autotune.set_launch(m, autotune.LANES, args.lanes)
-  if not args.no_tune:
+  lanes = lib.bgx_model_get_launch(m.handle, 3)
+  kernel = "k_mtree" if lanes > 1 else "k_rollout"
+  if lanes > 1:
+    # the multi-lane kernel's workgroup is one wavefront of 64 / lanes envs,
+    # and it keeps one contact row per candidate lane (no LDS tile)
+    epw = 64 // lanes
+  elif not args.no_tune:
     from brax_amd.generalized import autotune
     epw = autotune.tune_step(sys_, q_init, qd_init, acts[:2], N_FRAMES)
     # the tile trades occupancy against slow-path substeps, which grow as
@@ -280,14 +292,15 @@ def forward_bench(args, name):
                              "step, one HIP-graph rollout launch",
                  "envs": n, "env_steps": K, "substeps_per_env_step": N_FRAMES,
                  "qd_size": nv, "envs_per_workgroup": epw,
-                 "lanes_per_env": lib.bgx_model_get_launch(m.handle, 3),
-                 "fast_contact_slots": lib.bgx_model_get_launch(m.handle, 2),
+                 "lanes_per_env": lanes,
+                 "fast_contact_slots": (lib.bgx_model_get_launch(m.handle, 2)
+                                        if lanes == 1 else None),
                  "parallelism": "none (1 GPU)"},
       "roofline": {"bound": "valu", "achieved": achieved,
                    "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                    "frac": achieved / FP32_PEAK_TFLOPS,
-                   "traffic": measured_traffic(name, n, K),
-                   "traffic_source": f"profiles/k_rollout_{name}_{n}.md (rocprofv3 --pmc "
+                   "traffic": measured_traffic(kernel, name, n, K),
+                   "traffic_source": f"{profile_path(kernel, name, n)} (rocprofv3 --pmc "
                                      "FETCH_SIZE, WRITE_SIZE of this command)",
                    "flop_per_substep": fl, "mean_active_rows": r_a,
                    "mean_active_rows_source": r_a_src,
