@@ -344,7 +344,7 @@ def adjoint_cpu_baseline(sys_, q0, qd0, acts, n_frames=N_FRAMES):
   }
 
 
-def fwd_adjoint_leg(n, T, ckpt, warmup, tune=True, cpu=True):
+def fwd_adjoint_leg(n, T, ckpt, warmup, tune=True, cpu=True, parts=0):
   """Forward + adjoint of a T-step rollout of n quadrupeds (loss
   -sum_t v_torso,x + 1e-3 sum |act|^2, gradient w.r.t. every action and the
   initial qd, checkpoint every ckpt env steps), timed with HIP events on the
@@ -379,11 +379,15 @@ def fwd_adjoint_leg(n, T, ckpt, warmup, tune=True, cpu=True):
     st = pipeline.State(q=q0t, qd=qd0t, x=None, xd=None)
     return pipeline.rollout(sys_, st, acts0, N_FRAMES, trajectory=True, frames=False)
 
+  from brax_amd.generalized import autotune
+  m = pipeline.device_model(sys_, dev)
+  autotune.set_launch(m, autotune.ADJOINT_PARTS, parts)
   epw = [64, 64]
   if tune:
-    from brax_amd.generalized import autotune
     epw[0] = autotune.tune_step(sys_, q0t, qd0t, acts0[:2], N_FRAMES)
-    epw[1] = autotune.tune_adjoint(sys_, q0t, qd0t, acts0[:2], N_FRAMES)
+    # the backward pass over one checkpoint segment: a shorter window times
+    # the launch overheads rather than the kernels
+    epw[1] = autotune.tune_adjoint(sys_, q0t, qd0t, acts0[:ckpt], N_FRAMES)
   fwd_bwd()
   for _ in range(max(1, warmup // 50)):
     fwd_bwd()
@@ -412,6 +416,8 @@ def fwd_adjoint_leg(n, T, ckpt, warmup, tune=True, cpu=True):
       "forward_value": fwd_rate,
       "fwd_adjoint_over_forward": fb_rate / fwd_rate,
       "envs_per_workgroup": {"step": epw[0], "adjoint": epw[1]},
+      # env parts of the backward pass, one stream each (0: one)
+      "adjoint_parts": parts or 1,
       "finite": finite,
   }
   cpu_b, r_a = None, None
@@ -447,7 +453,7 @@ def apg_bench(args):
   n = args.envs or WORKLOADS["apg"][1]
   try:
     leg = fwd_adjoint_leg(n, args.horizon, args.ckpt, args.warmup, tune=not args.no_tune,
-                          cpu=not args.no_cpu)
+                          cpu=not args.no_cpu, parts=args.adj_parts)
   except _lib.BgxError as e:
     print(f"apg workload unavailable: {e}", file=sys.stderr)
     sys.exit(3)
@@ -576,6 +582,8 @@ def main():
   ap.add_argument("--lanes", type=int, default=0,
                   help="lanes per env of the step kernels (1: one thread per env; 0: the "
                        "topology's default)")
+  ap.add_argument("--adj-parts", type=int, default=0,
+                  help="env parts (streams) of the adjoint's backward pass, 1-4 (0: automatic)")
   ap.add_argument("--no-graph", action="store_true")
   ap.add_argument("--no-cpu", action="store_true")
   ap.add_argument("--no-adjoint", action="store_true",
@@ -602,7 +610,8 @@ def main():
       # the metric's second part: fwd+adjoint of the 4096-quadruped,
       # 128-step differentiable rollout with a checkpoint every 16 env steps
       out["fwd_adjoint"] = fwd_adjoint_leg(4096, 128, 16, args.warmup,
-                                           tune=not args.no_tune, cpu=not args.no_cpu)
+                                           tune=not args.no_tune, cpu=not args.no_cpu,
+                                           parts=args.adj_parts)
       if out["config"]["envs"] == 65536:
         out["metric"] = METRIC
   print(json.dumps(out))

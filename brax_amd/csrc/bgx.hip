@@ -21,6 +21,9 @@ BGX_FOR_EACH_TOPOLOGY(BGX_DECL_OPS)
 
 namespace {
 
+// most env parts of the adjoint's backward pass (one stream each)
+constexpr int kMaxAdjParts = 4;
+
 #define BGX_OPS_PTR(ID, T) bgx_ops_##T(),
 const bgx::TopoOps* const kOps[] = {BGX_FOR_EACH_TOPOLOGY(BGX_OPS_PTR)};
 #undef BGX_OPS_PTR
@@ -82,6 +85,7 @@ struct bgx_model {
   int epw[2];        // envs per workgroup: step kernels, adjoint kernels (0 = 64)
   int kc;            // fast contact slots of the step kernels (0: the default)
   int lanes;         // lanes per env of the step kernels (0: the default)
+  int adj_parts;     // env parts of the adjoint's backward pass (0: automatic)
 };
 
 extern "C" {
@@ -165,6 +169,14 @@ int bgx_model_set_launch(bgx_model* m, int kernels, int32_t envs_per_workgroup) 
     m->lanes = k;
     return BGX_OK;
   }
+  if (kernels == BGX_KERNELS_ADJOINT_PARTS) {
+    const int k = envs_per_workgroup;
+    if (k < 0 || k > kMaxAdjParts)
+      return fail(BGX_ERR_ARG, "adjoint parts must be 0 (automatic) to " +
+                                   std::to_string(kMaxAdjParts));
+    m->adj_parts = k;
+    return BGX_OK;
+  }
   if (kernels != BGX_KERNELS_STEP && kernels != BGX_KERNELS_ADJOINT)
     return fail(BGX_ERR_ARG, "unknown kernel group");
   const int e = envs_per_workgroup;
@@ -180,6 +192,7 @@ int bgx_model_get_launch(const bgx_model* m, int kernels) {
     const int w = kOps[m->topo]->mlanes;
     return m->lanes ? m->lanes : (w ? w : 1);
   }
+  if (m && kernels == BGX_KERNELS_ADJOINT_PARTS) return m->adj_parts;
   if (!m || (kernels != BGX_KERNELS_STEP && kernels != BGX_KERNELS_ADJOINT)) return -1;
   return m->epw[kernels] ? m->epw[kernels] : bgx::WAVE;
 }
@@ -367,14 +380,177 @@ int64_t bgx_adjoint_workspace_min_bytes(const bgx_model* m, int32_t n, int32_t c
 
 namespace {
 
+// Envs are independent in the adjoint as in the forward, so the backward
+// pass of a batch splits into parts of whole envs, each the same problem at
+// a smaller batch.  A part runs on its own stream: the reverse sweep (stage
+// A2) of one part keeps a few SIMDs busy for its whole length, and the
+// other parts' stage-B and stage-A1 waves fill the rest.  Every env's
+// arithmetic is the same in any split (the kernels see the same inputs), so
+// the results are bitwise those of one part.  It pays only while the
+// streams keep hardware queues of their own: where they share one (more
+// streams in the process than queues), the parts run one after another and
+// each pays the sweep's full dependent chain, so one part is the default.
+constexpr int kMaxAdjDevices = 64;
+
+struct AdjStreams {
+  hipStream_t s[kMaxAdjParts - 1];
+  hipEvent_t fork, join[kMaxAdjParts - 1];
+};
+std::mutex g_adj_mu;  // guards the table and orders the event records
+AdjStreams* g_adj_streams[kMaxAdjDevices] = {};
+
+// the helper streams and events of the current device, created on first use
+// (default priority; kept until the process ends); call with g_adj_mu held
+int adj_streams(AdjStreams** out) {
+  int dev = -1;
+  int rc = check_hip(hipGetDevice(&dev), "hipGetDevice");
+  if (rc) return rc;
+  if (dev < 0 || dev >= kMaxAdjDevices) return fail(BGX_ERR_ARG, "device index out of range");
+  if (!g_adj_streams[dev]) {
+    AdjStreams h{};
+    for (int i = 0; i < kMaxAdjParts - 1; ++i) {
+      if ((rc = check_hip(hipStreamCreateWithFlags(&h.s[i], hipStreamNonBlocking),
+                          "adjoint stream")))
+        return rc;
+      if ((rc = check_hip(hipEventCreateWithFlags(&h.join[i], hipEventDisableTiming),
+                          "adjoint event")))
+        return rc;
+    }
+    if ((rc = check_hip(hipEventCreateWithFlags(&h.fork, hipEventDisableTiming),
+                        "adjoint event")))
+      return rc;
+    g_adj_streams[dev] = new AdjStreams(h);
+  }
+  *out = g_adj_streams[dev];
+  return BGX_OK;
+}
+
+// parts of the backward pass for a batch of n envs (0 = one), each a
+// multiple of 64 envs but the last
+int adj_parts(int setting, int n, int* per) {
+  int p = setting > 0 ? setting : 1;
+  p = p < 1 ? 1 : (p > kMaxAdjParts ? kMaxAdjParts : p);
+  *per = ((n + p - 1) / p + 63) / 64 * 64;
+  return (n + *per - 1) / *per;
+}
+
+// the part of workspace w that holds envs [env0, env0 + nh) in the layout
+// of a batch of nh envs (each buffer is a multiple of n per-env blocks)
+AdjWorkspace part_view(const AdjWorkspace& w, const bgx::TopoOps* ops, int64_t k,
+                       int64_t n_frames, int64_t env0) {
+  const int64_t nq = ops->dims[1], nv = ops->dims[2];
+  AdjWorkspace p = w;
+  p.sub_q = w.sub_q + w.segs * k * n_frames * env0 * nq;
+  p.sub_qd = w.sub_qd + w.segs * k * n_frames * env0 * nv;
+  p.q = w.q + w.segs * env0 * nq;
+  p.qd = w.qd + w.segs * env0 * nv;
+  p.adj = w.adj + env0 * (nq + nv);
+  p.rec = w.rec + w.batch * (int64_t)ops->adj_rec * env0;
+  return p;
+}
+
+// dst (rows, w) <- rows of w floats of src, ss floats apart
+__global__ void k_copy_rows(float* dst, const float* src, long long rows, long long w,
+                            long long ss) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < rows * w) dst[i] = src[(i / w) * ss + i % w];
+}
+
+// The backward pass of envs [env0, env0 + nh) of a batch of n on stream s,
+// with that part's workspace w (laid out for nh envs).  The caller's arrays
+// keep the batch's layout: per-step blocks of n envs, the part's rows at
+// env0.
+int adjoint_part(const bgx_model* m, int n, int env0, int nh, int n_steps, int n_frames,
+                 int ckpt_every, const float* ckpt_q, const float* ckpt_qd, const float* act,
+                 int64_t act_step_stride, const float* gq_traj, const float* gqd_traj,
+                 float* gact, float* gq0, float* gqd0, const AdjWorkspace& w,
+                 int32_t* status, hipStream_t s) {
+  const bgx::TopoOps* ops = kOps[m->topo];
+  const int nq = ops->dims[1], nv = ops->dims[2], nu = ops->dims[3];
+  const float* act_p = act ? act + (long long)env0 * nu : nullptr;
+  int rc;
+  const int n_seg = (n_steps + ckpt_every - 1) / ckpt_every;
+  for (int seg_hi = n_seg - 1; seg_hi >= 0;) {
+    // a group of segments, recomputed side by side by one launch (a short
+    // last segment goes alone)
+    const bool short_last = seg_hi == n_seg - 1 && n_steps % ckpt_every != 0;
+    const int cnt = short_last ? 1 : (seg_hi + 1 < (int)w.segs ? seg_hi + 1 : (int)w.segs);
+    const int seg_lo = seg_hi - cnt + 1;
+    const int ns_group = short_last ? n_steps - seg_hi * ckpt_every : ckpt_every;
+    hipLaunchKernelGGL(k_copy_rows, grid1((long long)cnt * nh * nq), dim3(256), 0, s, w.q,
+                       ckpt_q + ((long long)seg_lo * n + env0) * nq, (long long)cnt,
+                       (long long)nh * nq, (long long)n * nq);
+    hipLaunchKernelGGL(k_copy_rows, grid1((long long)cnt * nh * nv), dim3(256), 0, s, w.qd,
+                       ckpt_qd + ((long long)seg_lo * n + env0) * nv, (long long)cnt,
+                       (long long)nh * nv, (long long)n * nv);
+    bgx::Launch f{};
+    f.M = m->table, f.n = nh, f.n_steps = ns_group, f.n_frames = n_frames;
+    f.q = w.q, f.qd = w.qd;
+    f.act = act_p ? act_p + (long long)seg_lo * ckpt_every * act_step_stride : nullptr;
+    f.act_stride = act_step_stride;
+    f.n_seg = cnt, f.seg_act_stride = (long long)ckpt_every * act_step_stride;
+    f.sub_q = w.sub_q, f.sub_qd = w.sub_qd;
+    f.status = status ? status + env0 : nullptr, f.status_or = 1;
+    f.epw = m->epw[BGX_KERNELS_ADJOINT];
+    f.kc = m->kc;  // the forward's tile and kernel: the recompute repeats its states
+    f.lanes = m->lanes;
+    if ((rc = check_hip(ops->rollout(f, s), "recompute launch"))) return rc;
+    for (int seg = seg_hi; seg >= seg_lo; --seg) {
+      const int s0 = seg * ckpt_every;
+      // substep states of this segment (the launch's segment stride is its
+      // own length ns_group)
+      const long long sub_off = (long long)(seg - seg_lo) * ns_group * n_frames * nh;
+      float* sq = w.sub_q + sub_off * nq;
+      float* sqd = w.sub_qd + sub_off * nv;
+      // substeps of the segment, last first, taped in batches of w.batch
+      const int n_sub = ns_group * n_frames;
+      const long long rec_stride = (long long)ops->adj_rec * nh;
+      for (int hi = n_sub - 1; hi >= 0; hi -= (int)w.batch) {
+        const int lo = hi + 1 - (int)w.batch > 0 ? hi + 1 - (int)w.batch : 0;
+        bgx::AdjLaunch t{};
+        t.M = m->table, t.n = nh, t.epw = m->epw[BGX_KERNELS_ADJOINT];
+        t.q = sq, t.qd = sqd;
+        t.act = act_p ? act_p + (long long)s0 * act_step_stride : nullptr;
+        t.act_stride = act_step_stride, t.n_frames = n_frames;
+        t.sub_lo = lo, t.n_sub = hi - lo + 1;
+        t.rec = w.rec, t.rec_stride = rec_stride;
+        if ((rc = check_hip(ops->adj_fwd(t, s), "adjoint stage A1 launch"))) return rc;
+        for (int i = hi; i >= lo; --i) {
+          const int t_step = s0 + i / n_frames;
+          const bool step_end = (i % n_frames) == n_frames - 1;
+          const long long row = (long long)t_step * n + env0;
+          bgx::AdjLaunch a{};
+          a.M = m->table, a.n = nh, a.epw = m->epw[BGX_KERNELS_ADJOINT];
+          a.q = sq + (long long)i * nh * nq;
+          a.qd = sqd + (long long)i * nh * nv;
+          a.act = act_p ? act_p + (long long)t_step * act_step_stride : nullptr;
+          a.gq_add = (step_end && gq_traj) ? gq_traj + row * nq : nullptr;
+          a.gqd_add = (step_end && gqd_traj) ? gqd_traj + row * nv : nullptr;
+          a.adj = w.adj, a.rec = w.rec + (long long)(i - lo) * rec_stride;
+          a.gact = gact ? gact + row * nu : nullptr;
+          if ((rc = check_hip(ops->adj_rev(a, s), "adjoint stage A2 launch"))) return rc;
+          if ((rc = check_hip(ops->adj_b(a, s), "adjoint stage B launch"))) return rc;
+        }
+      }
+    }
+    seg_hi = seg_lo - 1;
+  }
+  hipLaunchKernelGGL(k_cols, grid1((long long)nh * nq), dim3(256), 0, s,
+                     gq0 + (long long)env0 * nq, w.adj, (long long)nh, nq, nq + nv, 0);
+  hipLaunchKernelGGL(k_cols, grid1((long long)nh * nv), dim3(256), 0, s,
+                     gqd0 + (long long)env0 * nv, w.adj, (long long)nh, nv, nq + nv, nq);
+  return check_hip(hipGetLastError(), "output copy");
+}
+
 // The adjoint of a rollout with its workspace at `workspace` (shared by the
-// workspace-argument entry point and the no-allocation one).
+// workspace-argument entry point and the no-allocation one; split_streams
+// false keeps every launch on the caller's stream and creates nothing).
 int rollout_adjoint_impl(const bgx_model* m, int32_t n, int32_t n_steps, int32_t n_frames,
                          int32_t ckpt_every, const float* ckpt_q, const float* ckpt_qd,
                          const float* act, int64_t act_step_stride, const float* gq_traj,
                          const float* gqd_traj, float* gact, float* gq0, float* gqd0,
                          float* workspace, int64_t workspace_bytes, int32_t* status,
-                         void* stream) {
+                         void* stream, bool split_streams) {
   if (!m) return fail(BGX_ERR_ARG, "model is null");
   if (!valid_topo(m->topo)) return fail(BGX_ERR_TOPOLOGY, "unknown topology id");
   if (n < 0 || n_steps < 0 || n_frames < 1 || ckpt_every < 1)
@@ -415,74 +591,35 @@ int rollout_adjoint_impl(const bgx_model* m, int32_t n, int32_t n_steps, int32_t
   if (status) hipLaunchKernelGGL(k_fill_i, grid1(n), dim3(256), 0, s, status, (long long)n, 0);
   if ((rc = check_hip(hipGetLastError(), "fill"))) return rc;
 
-  const int n_seg = (n_steps + ckpt_every - 1) / ckpt_every;
-  for (int seg_hi = n_seg - 1; seg_hi >= 0;) {
-    // a group of segments, recomputed side by side by one launch (a short
-    // last segment goes alone)
-    const bool short_last = seg_hi == n_seg - 1 && n_steps % ckpt_every != 0;
-    const int cnt = short_last ? 1 : (seg_hi + 1 < (int)w.segs ? seg_hi + 1 : (int)w.segs);
-    const int seg_lo = seg_hi - cnt + 1;
-    const int ns_group = short_last ? n_steps - seg_hi * ckpt_every : ckpt_every;
-    hipLaunchKernelGGL(k_copy, grid1((long long)cnt * n * nq), dim3(256), 0, s, w.q,
-                       ckpt_q + (long long)seg_lo * n * nq, (long long)cnt * n * nq);
-    hipLaunchKernelGGL(k_copy, grid1((long long)cnt * n * nv), dim3(256), 0, s, w.qd,
-                       ckpt_qd + (long long)seg_lo * n * nv, (long long)cnt * n * nv);
-    bgx::Launch f{};
-    f.M = m->table, f.n = n, f.n_steps = ns_group, f.n_frames = n_frames;
-    f.q = w.q, f.qd = w.qd;
-    f.act = act ? act + (long long)seg_lo * ckpt_every * act_step_stride : nullptr;
-    f.act_stride = act_step_stride;
-    f.n_seg = cnt, f.seg_act_stride = (long long)ckpt_every * act_step_stride;
-    f.sub_q = w.sub_q, f.sub_qd = w.sub_qd;
-    f.status = status, f.status_or = 1;
-    f.epw = m->epw[BGX_KERNELS_ADJOINT];
-    f.kc = m->kc;  // the forward's tile and kernel: the recompute repeats its states
-    f.lanes = m->lanes;
-    if ((rc = check_hip(ops->rollout(f, s), "recompute launch"))) return rc;
-    for (int seg = seg_hi; seg >= seg_lo; --seg) {
-      const int s0 = seg * ckpt_every;
-      // substep states of this segment (the launch's segment stride is its
-      // own length ns_group)
-      const long long sub_off = (long long)(seg - seg_lo) * ns_group * n_frames * n;
-      float* sq = w.sub_q + sub_off * nq;
-      float* sqd = w.sub_qd + sub_off * nv;
-      // substeps of the segment, last first, taped in batches of w.batch
-      const int n_sub = ns_group * n_frames;
-      const long long rec_stride = (long long)ops->adj_rec * n;
-      for (int hi = n_sub - 1; hi >= 0; hi -= (int)w.batch) {
-        const int lo = hi + 1 - (int)w.batch > 0 ? hi + 1 - (int)w.batch : 0;
-        bgx::AdjLaunch t{};
-        t.M = m->table, t.n = n, t.epw = m->epw[BGX_KERNELS_ADJOINT];
-        t.q = sq, t.qd = sqd;
-        t.act = act ? act + (long long)s0 * act_step_stride : nullptr;
-        t.act_stride = act_step_stride, t.n_frames = n_frames;
-        t.sub_lo = lo, t.n_sub = hi - lo + 1;
-        t.rec = w.rec, t.rec_stride = rec_stride;
-        if ((rc = check_hip(ops->adj_fwd(t, s), "adjoint stage A1 launch"))) return rc;
-        for (int i = hi; i >= lo; --i) {
-          const int t_step = s0 + i / n_frames;
-          const bool step_end = (i % n_frames) == n_frames - 1;
-          bgx::AdjLaunch a{};
-          a.M = m->table, a.n = n, a.epw = m->epw[BGX_KERNELS_ADJOINT];
-          a.q = sq + (long long)i * n * nq;
-          a.qd = sqd + (long long)i * n * nv;
-          a.act = act ? act + (long long)t_step * act_step_stride : nullptr;
-          a.gq_add = (step_end && gq_traj) ? gq_traj + (long long)t_step * n * nq : nullptr;
-          a.gqd_add = (step_end && gqd_traj) ? gqd_traj + (long long)t_step * n * nv : nullptr;
-          a.adj = w.adj, a.rec = w.rec + (long long)(i - lo) * rec_stride;
-          a.gact = gact ? gact + (long long)t_step * n * nu : nullptr;
-          if ((rc = check_hip(ops->adj_rev(a, s), "adjoint stage A2 launch"))) return rc;
-          if ((rc = check_hip(ops->adj_b(a, s), "adjoint stage B launch"))) return rc;
-        }
-      }
-    }
-    seg_hi = seg_lo - 1;
+  // the parts, on the caller's stream and up to kMaxAdjParts - 1 helper
+  // streams forked from it and joined back into it
+  int per = n;
+  const int parts = split_streams ? adj_parts(m->adj_parts, n, &per) : 1;
+  if (parts == 1)
+    return adjoint_part(m, n, 0, n, n_steps, n_frames, ckpt_every, ckpt_q, ckpt_qd, act,
+                        act_step_stride, gq_traj, gqd_traj, gact, gq0, gqd0, w, status, s);
+  std::lock_guard<std::mutex> lock(g_adj_mu);
+  AdjStreams* hs = nullptr;
+  if ((rc = adj_streams(&hs))) return rc;
+  if ((rc = check_hip(hipEventRecord(hs->fork, s), "adjoint fork"))) return rc;
+  for (int p = 1; p < parts; ++p)
+    if ((rc = check_hip(hipStreamWaitEvent(hs->s[p - 1], hs->fork, 0), "adjoint fork")))
+      return rc;
+  for (int p = 0; p < parts && !rc; ++p) {
+    const int env0 = p * per, nh = n - env0 < per ? n - env0 : per;
+    rc = adjoint_part(m, n, env0, nh, n_steps, n_frames, ckpt_every, ckpt_q, ckpt_qd, act,
+                      act_step_stride, gq_traj, gqd_traj, gact, gq0, gqd0,
+                      part_view(w, ops, ckpt_every, n_frames, env0), status,
+                      p ? hs->s[p - 1] : s);
   }
-  hipLaunchKernelGGL(k_cols, grid1((long long)n * nq), dim3(256), 0, s, gq0, w.adj,
-                     (long long)n, nq, nq + nv, 0);
-  hipLaunchKernelGGL(k_cols, grid1((long long)n * nv), dim3(256), 0, s, gqd0, w.adj,
-                     (long long)n, nv, nq + nv, nq);
-  return check_hip(hipGetLastError(), "output copy");
+  // join every helper, also after a failed launch, so that the caller's
+  // stream never runs ahead of work in flight on the workspace
+  for (int p = 1; p < parts; ++p) {
+    const int r1 = check_hip(hipEventRecord(hs->join[p - 1], hs->s[p - 1]), "adjoint join");
+    const int r2 = check_hip(hipStreamWaitEvent(s, hs->join[p - 1], 0), "adjoint join");
+    if (!rc) rc = r1 ? r1 : r2;
+  }
+  return rc;
 }
 
 // Workspace of bgx_rollout_adjoint_noalloc: one array per GPU in this file's
@@ -529,7 +666,7 @@ int bgx_rollout_adjoint(const bgx_model* m, int32_t n, int32_t n_steps, int32_t 
                         void* stream) {
   return rollout_adjoint_impl(m, n, n_steps, n_frames, ckpt_every, ckpt_q, ckpt_qd, act,
                               act_step_stride, gq_traj, gqd_traj, gact, gq0, gqd0, workspace,
-                              workspace_bytes, status, stream);
+                              workspace_bytes, status, stream, true);
 }
 
 int64_t bgx_adjoint_scratch_bytes(void) { return kScratchBytes; }
@@ -715,7 +852,7 @@ int bgx_rollout_adjoint_noalloc(const bgx_model* m, int32_t n, int32_t n_steps,
   if (rc) return rc;
   return rollout_adjoint_impl(m, n, n_steps, n_frames, ckpt_every, ckpt_q, ckpt_qd, act,
                               act_step_stride, gq_traj, gqd_traj, gact, gq0, gqd0, ws,
-                              kScratchBytes, status, stream);
+                              kScratchBytes, status, stream, false);
 }
 
 }  // extern "C"

@@ -23,10 +23,13 @@
 //    quantities J is built from (cdof, the slots' lever arms and pyramid
 //    directions), a few floats per dof and per slot.  The row vectors of the
 //    sweep live in LDS; the lane pair splits each iteration's products.
-//  B (one thread per env and input coordinate): run the same formulas that
+//  B (one thread per env and q coordinate): run the same formulas that
 //    build those ingredients from (q, qd) in dual numbers along one input
 //    coordinate and contract the tangents with the cotangents of stage A:
-//    d/de <cotangents, ingredients(q + e dq, qd + e dqd)>.
+//    d/de <cotangents, ingredients(q + e dq, qd)>.  The qd coordinates take
+//    one more thread per env: only qf_smooth and aref move with qd, and one
+//    reverse pass through them (adjoint_b_qd_all) gives the gradient in all
+//    of qd at once.
 //
 // Record of stages A1/A2 for stage B: one float per field per env,
 // field-major (field f of env e at rec[f * n_envs + e]).
@@ -621,17 +624,11 @@ BGX_D float adjoint_b_q(const float* __restrict__ M, const float* q_in, const fl
   float phi = 0.f;
 #pragma unroll
   for (int d = 0; d < NV; ++d) phi += rec[R::QFB + d] * qf[d].d;
-  {
-    S Ml[NV][NV];
-    crba<T>(M, cc, ci, ca, cv, [&](int i, int j, S v) { Ml[i][j] = v; });
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-      phi += rec[R::MB + mpat_idx<T>(i, i)] * Ml[i][i].d;
-#pragma unroll
-      for (int j = 0; j < i; ++j)
-        if ((T::DANC[i] >> j) & 1u) phi += rec[R::MB + mpat_idx<T>(i, j)] * Ml[i][j].d;
-    }
-  }
+  // crba emits each entry of the dof-ancestor pattern once (diagonal, then
+  // the row's ancestors in ascending order): contract it as it comes instead
+  // of holding the dual matrix
+  crba<T>(M, cc, ci, ca, cv,
+          [&](int i, int j, S v) { phi += rec[R::MB + mpat_idx<T>(i, j)] * v.d; });
 
   // contact rows of the active candidates
 #pragma unroll
@@ -831,6 +828,216 @@ BGX_D float adjoint_b_qd(const float* __restrict__ M, const float* q_in, const f
     }
   }
   return phi + rec[R::GQDT + j];
+}
+
+// Along every qd coordinate at once: the gradient in qd of <cotangents,
+// ingredients(q, qd)> by one reverse pass, where adjoint_b_qd takes one
+// coordinate at a time in dual numbers.  Only qf_smooth and aref move with
+// qd: qf_smooth through the link velocities (link_vel) and RNEA's velocity
+// terms (rnea_bias: the cdofd cross terms and cd x* (I cd)), the damping and
+// the actuators' velocity bias (smooth_forces); aref through the contact
+// points' and limited dofs' velocities.  Transposes used: inertia_mul is a
+// symmetric map; cross_motion(s, .)^T = -cross_force(s, .); cross_force
+// taken as linear in s has the transpose y -> (f_a x y_a + f_v x y_v,
+// f_v x y_a).  Writes d phi / d qd_j (with the pass-through) to gqd[j].
+template <class T>
+BGX_D void adjoint_b_qd_all(const float* __restrict__ M, const float* q_in,
+                            const float* qd_in, const float* act, RecPtr rec, float* gqd) {
+  using R = AdjRec<T>;
+  constexpr int NV = T::NV, L = T::L, NRC = R::NRC;
+  constexpr bool PZ = T::PLANE_Z != 0;
+  V3 xp[L];
+  Q4 xr[L];
+  forward_kinematics<T, false, float>(M, q_in, qd_in, xp, xr, nullptr, nullptr);
+  V3 rc[L];
+  root_coms<T>(M, xp, xr, rc);
+  V3 cc[L], ca[NV], cv[NV];
+  Sym3 ci[L];
+  com_inertia<T>(M, xp, xr, rc, cc, ci);
+  dof_motions<T>(M, q_in, xp, xr, rc, ca, cv);
+  V3 cda[L], cdv[L];
+  link_vel<T>(ca, cv, qd_in, cda, cdv);
+  float qfb[NV], g[NV];
+#pragma unroll
+  for (int d = 0; d < NV; ++d) {
+    qfb[d] = rec[R::QFB + d];
+    g[d] = rec[R::GQDT + d] - dof_rec<T>(M, d)[DF_DAMPING] * qfb[d];
+  }
+  // actuators' velocity bias, where the force is not clipped
+#pragma unroll
+  for (int a = 0; a < T::NU; ++a) {
+    const float* ac = M + T::OFF_ACT + a * ACT_STRIDE;
+    const int qa = T::ACT_Q[a], da = T::ACT_D[a];
+    const float ctl = clipf(act[a], ac[AC_CTRL_LO], ac[AC_CTRL_HI]);
+    const float f = ac[AC_GAIN] * ctl +
+                    ac[AC_GEAR] * (q_in[qa] * ac[AC_BIAS_Q] + qd_in[da] * ac[AC_BIAS_QD]);
+    if (f >= ac[AC_FORCE_LO] && f <= ac[AC_FORCE_HI])
+      g[da] += qfb[da] * ac[AC_GEAR] * ac[AC_GEAR] * ac[AC_BIAS_QD];
+  }
+  // cotangent of each link's RNEA force: qf_smooth = ... - bias, bias_d =
+  // cdof_d . (subtree sum of forces), so the force of link l carries
+  // -sum qfb_d cdof_d over the dofs of l and its ancestors (root to leaf)
+  V3 fba[L], fbv[L];
+#pragma unroll
+  for (int l = 0; l < L; ++l) {
+    const int par = T::PARENT[l];
+    V3 a = par >= 0 ? fba[par] : v3zero(), v = par >= 0 ? fbv[par] : v3zero();
+    const int nd = T::LTYPE[l] == 0 ? 6 : T::LTYPE[l];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      if (k < nd) {
+        const int d = T::DADR[l] + k;
+        a = a - ca[d] * qfb[d];
+        v = v - cv[d] * qfb[d];
+      }
+    }
+    fba[l] = a, fbv[l] = v;
+  }
+  // cotangents of the link velocities (cba, cbv) and of RNEA's
+  // accelerations (xba, xbv)
+  V3 cba[L], cbv[L], xba[L], xbv[L];
+#pragma unroll
+  for (int l = 0; l < L; ++l) cba[l] = cbv[l] = xba[l] = xbv[l] = v3zero();
+  // aref's velocity terms of the active contacts: aref = -b (d . v_contact)
+  // - k imp dist over the pyramid rows
+#pragma unroll
+  for (int c = 0; c < T::NCAND; ++c) {
+    const int s = tape_int(rec[R::CSLOT + c], -1, R::KS - 1);
+    if (s >= 0) {
+      const int l = T::CAND_LINK[c], la = T::CAND_LINKA[c];
+      const float* cd = cand_rec<T>(M, c);
+      const int fb = tape_int(rec[R::CFB + c], 0, 1);
+      V3 ctr, axis, pos, n, t1, t2;
+      float dist;
+      if (T::PAIRS && T::CAND_KIND[c] >= KIND_SS) {
+        pair_contact<T>(M, c, xp, xr, fb, dist, pos, n, t1, t2);
+        if (T::CAND_FLIP[c]) n = n * -1.f, t1 = t1 * -1.f, t2 = t2 * -1.f;
+      } else {
+        candidate_center<T>(M, c, xp, xr, ctr, axis);
+        dist = candidate_dist<T>(M, c, ctr);
+        contact_frame<T>(M, c, ctr, axis, dist, fb, pos, t1, t2);
+        n = PZ ? v3(0.f, 0.f, 1.f) : ldv3(cd + CD_PN);
+      }
+      const float mu = cd[CD_MU];
+      const float bq = cd[CD_IMP + IM_B];
+      const int r0 = 4 * s;
+      const float a0 = rec[R::AB + r0], a1 = rec[R::AB + r0 + 1], a2 = rec[R::AB + r0 + 2],
+                  a3 = rec[R::AB + r0 + 3];
+      const V3 vb = (n * ((a0 + a1) + (a2 + a3)) + t1 * (mu * (a0 - a1)) +
+                     t2 * (mu * (a2 - a3))) * -bq;
+      cba[l] = cba[l] + cross(pos - rc[l], vb);
+      cbv[l] = cbv[l] + vb;
+      if (la >= 0) {
+        const int lc = la >= 0 ? la : 0;
+        cba[lc] = cba[lc] - cross(pos - rc[lc], vb);
+        cbv[lc] = cbv[lc] - vb;
+      }
+    }
+  }
+  // leaf to root: each link's force, its accelerations' dof terms, then its
+  // velocity's dof terms; a link's cotangents are final once its children
+  // (higher indices) have added theirs
+#pragma unroll
+  for (int l = L - 1; l >= 0; --l) {
+    const int par = T::PARENT[l], da = T::DADR[l];
+    const float m = link_rec<T>(M, l)[LK_MASS];
+    // f_l = I x_l + cd_l x* (I cd_l)
+    V3 ta, tv;
+    inertia_mul(cc[l], ci[l], m, fba[l], fbv[l], ta, tv);
+    xba[l] = xba[l] + ta, xbv[l] = xbv[l] + tv;
+    V3 ha, hv;
+    inertia_mul(cc[l], ci[l], m, cda[l], cdv[l], ha, hv);
+    cba[l] = cba[l] + (cross(ha, fba[l]) + cross(hv, fbv[l]));
+    cbv[l] = cbv[l] + cross(hv, fba[l]);
+    {
+      V3 ma, mv, na, nv;
+      cross_motion(cda[l], cdv[l], fba[l], fbv[l], ma, mv);
+      inertia_mul(cc[l], ci[l], m, ma, mv, na, nv);
+      cba[l] = cba[l] - na, cbv[l] = cbv[l] - nv;
+    }
+    // x_l = x_parent + sum_k cross_motion(s_k, cdof_d) qd_d
+    if (T::LTYPE[l] == 0) {
+      V3 sa = v3zero(), sv = v3zero();
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        sa = sa + ca[da + k] * qd_in[da + k];
+        sv = sv + cv[da + k] * qd_in[da + k];
+      }
+      V3 sba = v3zero(), sbv = v3zero();
+#pragma unroll
+      for (int k = 3; k < 6; ++k) {
+        const int d = da + k;
+        V3 oa, ov, pa, pv;
+        cross_motion(sa, sv, ca[d], cv[d], oa, ov);
+        g[d] += dot(xba[l], oa) + dot(xbv[l], ov);
+        cross_force(ca[d], cv[d], xba[l], xbv[l], pa, pv);
+        sba = sba + pa * qd_in[d], sbv = sbv + pv * qd_in[d];
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k) g[da + k] += dot(sba, ca[da + k]) + dot(sbv, cv[da + k]);
+    } else {
+      constexpr int NJ = 3;
+      V3 sa = par >= 0 ? cda[par >= 0 ? par : 0] : v3zero();
+      V3 sv = par >= 0 ? cdv[par >= 0 ? par : 0] : v3zero();
+      V3 sba[NJ], sbv[NJ];
+#pragma unroll
+      for (int k = 0; k < NJ; ++k) {
+        sba[k] = sbv[k] = v3zero();
+        if (k < T::LTYPE[l]) {
+          const int d = da + k;
+          if (k > 0) sa = sa + ca[d - 1] * qd_in[d - 1], sv = sv + cv[d - 1] * qd_in[d - 1];
+          V3 oa, ov, pa, pv;
+          cross_motion(sa, sv, ca[d], cv[d], oa, ov);
+          g[d] += dot(xba[l], oa) + dot(xbv[l], ov);
+          cross_force(ca[d], cv[d], xba[l], xbv[l], pa, pv);
+          sba[k] = pa * qd_in[d], sbv[k] = pv * qd_in[d];
+        }
+      }
+      // s_k = cd_parent + sum_{j<k} cdof_j qd_j
+      V3 aa = v3zero(), av = v3zero();
+#pragma unroll
+      for (int k = NJ - 1; k >= 0; --k) {
+        if (k < T::LTYPE[l]) {
+          if (k + 1 < T::LTYPE[l])
+            g[da + k] += dot(aa, ca[da + k]) + dot(av, cv[da + k]);
+          aa = aa + sba[k], av = av + sbv[k];
+        }
+      }
+      if (par >= 0) {
+        const int p = par >= 0 ? par : 0;
+        cba[p] = cba[p] + aa, cbv[p] = cbv[p] + av;
+      }
+    }
+    // cd_l = cd_parent + sum cdof_d qd_d
+    const int nd = T::LTYPE[l] == 0 ? 6 : T::LTYPE[l];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      if (k < nd) {
+        const int d = da + k;
+        g[d] += dot(cba[l], ca[d]) + dot(cbv[l], cv[d]);
+      }
+    }
+    if (par >= 0) {
+      const int p = par >= 0 ? par : 0;
+      cba[p] = cba[p] + cba[l], cbv[p] = cbv[p] + cbv[l];
+      xba[p] = xba[p] + xba[l], xbv[p] = xbv[p] + xbv[l];
+    }
+  }
+  // limit rows: aref = -b side qd_d - k imp pos
+#pragma unroll
+  for (int i = 0; i < T::NLIM; ++i) {
+    const int d = T::LIM_D[i], qi = T::LIM_Q[i];
+    const float* df = dof_rec<T>(M, d);
+    const float pmin = q_in[qi] - df[DF_LO];
+    const float pmax = df[DF_HI] - q_in[qi];
+    const float pos = fminf(fminf(pmin, pmax), 0.f);
+    if (pos < 0.f) {
+      const float side = pmin < pmax ? 1.f : -1.f;
+      g[d] += rec[R::AB + NRC + i] * (-df[DF_IMP + IM_B] * side);
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < NV; ++d) gqd[d] = g[d];
 }
 
 // one tree on a free root over a z-up plane: moving the root along x or y

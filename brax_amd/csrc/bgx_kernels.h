@@ -399,21 +399,31 @@ __global__ __launch_bounds__(WAVE) void k_adj_rev(AdjLaunch a) {
   }
 }
 
-// adjoint stage B: one thread per (env, input coordinate), coordinate-major
-// over env blocks of WAVE (a wave shares its coordinate; tape reads coalesce)
+// adjoint stage B: one thread per (env, q coordinate), coordinate-major over
+// env blocks of WAVE (a wave shares its coordinate; tape reads coalesce),
+// and after them one thread per env for all qd coordinates (one reverse
+// pass)
 template <class T>
 __global__ __launch_bounds__(WAVE) void k_adj_b(AdjLaunch a) {
   constexpr int NQ = T::NQ, NV = T::NV, ND = NQ + NV;
   const int n_pad = (a.n + WAVE - 1) / WAVE * WAVE;
   const long long g = (long long)blockIdx.x * WAVE + threadIdx.x;
   const int dir = (int)(g / n_pad), env = (int)(g % n_pad);
-  if (dir >= ND || env >= a.n) return;
+  if (dir > NQ || env >= a.n) return;
   float act[T::NU > 0 ? T::NU : 1];
 #pragma unroll
   for (int k = 0; k < T::NU; ++k) act[k] = a.act ? a.act[(long long)env * T::NU + k] : 0.f;
-  const float r = adjoint_b<T>(a.M, a.q + (long long)env * NQ, a.qd + (long long)env * NV,
-                               act, RecPtr{a.rec + env, a.n}, dir);
-  a.adj[(long long)env * ND + dir] = r;
+  const float* q = a.q + (long long)env * NQ;
+  const float* qd = a.qd + (long long)env * NV;
+  float* out = a.adj + (long long)env * ND;
+  if (dir < NQ) {
+    out[dir] = adjoint_b<T>(a.M, q, qd, act, RecPtr{a.rec + env, a.n}, dir);
+  } else {
+    float gqd[NV];
+    adjoint_b_qd_all<T>(a.M, q, qd, act, RecPtr{a.rec + env, a.n}, gqd);
+#pragma unroll
+    for (int j = 0; j < NV; ++j) out[NQ + j] = gqd[j];
+  }
 }
 
 // a batch launch of the rollout kernel: a persistent grid of `grid`
@@ -520,7 +530,8 @@ struct Ops {
     return hipGetLastError();
   }
   static hipError_t adj_b(const AdjLaunch& a, hipStream_t s) {
-    const long long threads = (long long)(a.n + WAVE - 1) / WAVE * WAVE * (T::NQ + T::NV);
+    // NQ q coordinates, then the qd reverse pass
+    const long long threads = (long long)(a.n + WAVE - 1) / WAVE * WAVE * (T::NQ + 1);
     hipLaunchKernelGGL(k_adj_b<T>, dim3((unsigned)((threads + WAVE - 1) / WAVE)), dim3(WAVE),
                        0, s, a);
     return hipGetLastError();

@@ -30,6 +30,7 @@ STEP = 0
 ADJOINT = 1
 TILE = 2  # the step kernels' LDS tile: fast contact slots per env
 LANES = 3  # the step kernels' lanes per env (1, or the multi-lane width)
+ADJOINT_PARTS = 4  # env parts of the adjoint's backward pass, a stream each (0: one)
 CANDIDATES: Tuple[int, ...] = (64, 32, 16, 8, 4)
 
 _CACHE: Dict[tuple, int] = {}

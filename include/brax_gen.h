@@ -99,6 +99,16 @@ int bgx_model_topology(const bgx_model* model);
  * env on a group of lanes (a lane per dof).  The two agree within rounding;
  * the getter returns the width in use. */
 #define BGX_KERNELS_LANES 3
+/* kernels = BGX_KERNELS_ADJOINT_PARTS splits bgx_rollout_adjoint's backward
+ * pass into that many parts of whole envs (1-4; 0 = one), each on its own
+ * stream forked from and joined back into the caller's (streams and events
+ * created on a device's first split call, default priority, kept until the
+ * process ends).  Results are bitwise the same for every split; the getter
+ * returns the setting.  The split is faster only while its streams get
+ * hardware queues of their own (few other streams in the process); where
+ * they share queues it is slower.  bgx_rollout_adjoint_noalloc always runs
+ * one part on the caller's stream. */
+#define BGX_KERNELS_ADJOINT_PARTS 4
 int bgx_model_set_launch(bgx_model* model, int kernels, int32_t envs_per_workgroup);
 /* the current setting (64 for 0), or -1 on bad arguments */
 int bgx_model_get_launch(const bgx_model* model, int kernels);

@@ -9,7 +9,9 @@
 // inputs.bin: float32 model table (T::SIZE), q (NQ), qd (NV), act (NU).
 //
 // It tapes one substep (stage A1, substep_ks with a slot per candidate),
-// runs A2 and B on that tape, then on copies of it whose integer fields --
+// runs A2 and B (each q coordinate in dual numbers, each qd coordinate in
+// dual numbers and all of qd by the reverse pass) on that tape, then on
+// copies of it whose integer fields --
 // active-contact count, solver iterations, each slot's link, each
 // candidate's slot and frame flag -- hold NaN, infinities, huge, negative
 // and just-out-of-range values.  Prints one line per case; exits 0 when
@@ -55,6 +57,10 @@ static int run_reverse(const float* M, const std::vector<float>& q,
     finite += std::isfinite(adjoint_b<T>(M, q.data(), qd.data(), act.data(),
                                          RecPtr{rec.data(), 1}, d)) ? 1 : 0;
   for (int k = 0; k < T::NU; ++k) finite += std::isfinite(gact[k]) ? 1 : 0;
+  // the qd gradient by one reverse pass (what the kernel runs for qd)
+  std::vector<float> gqd(T::NV);
+  adjoint_b_qd_all<T>(M, q.data(), qd.data(), act.data(), RecPtr{rec.data(), 1}, gqd.data());
+  for (int j = 0; j < T::NV; ++j) finite += std::isfinite(gqd[j]) ? 1 : 0;
   return finite;
 }
 
@@ -85,9 +91,10 @@ int main(int argc, char** argv) {
   const int n_c = (int)rec[R::NC], iters = (int)rec[R::ITERS];
   std::vector<float> ok = rec;
   const int f0 = run_reverse(M, q, qd, act, ok);
+  constexpr int n_out = T::NQ + 2 * T::NV + T::NU;
   std::printf("valid tape: %d active contacts, %d solver iterations, %d of %d outputs finite\n",
-              n_c, iters, f0, T::NQ + T::NV + T::NU);
-  if (f0 != T::NQ + T::NV + T::NU) return 3;
+              n_c, iters, f0, n_out);
+  if (f0 != n_out) return 3;
 
   const float inf = std::numeric_limits<float>::infinity();
   const float bad[] = {std::numeric_limits<float>::quiet_NaN(), inf, -inf, 1e30f, -1e30f,

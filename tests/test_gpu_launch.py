@@ -82,6 +82,35 @@ def test_adjoint_bitwise_across_envs_per_workgroup():
   assert np.isfinite(out[0][0]).all()
 
 
+@pytest.mark.parametrize("name,n,parts", [("quadruped", 300, 4), ("humanoid", 130, 2)])
+def test_adjoint_bitwise_across_stream_parts(name, n, parts):
+  """The adjoint's backward pass split into env parts on several streams
+  (300 quadrupeds: parts of 128, 128 and 44 envs; 130 humanoids: 128 and 2)
+  gives the gradients of one part bit for bit."""
+  sys = getattr(S, name)(0)
+  q0, qd0, acts = _inputs(sys, n, 6, seed=5)
+  if name == "humanoid":
+    acts = 0.4 * acts
+  m = pipeline.device_model(sys, DEV)
+  out = []
+  for k in (1, parts):
+    autotune.set_launch(m, autotune.ADJOINT_PARTS, k)
+    try:
+      q = q0.clone().requires_grad_(True)
+      qd = qd0.clone().requires_grad_(True)
+      a = acts.clone().requires_grad_(True)
+      qt, qdt, _ = pipeline.rollout_grad(sys, q, qd, a, 5, ckpt_every=4)
+      (-qdt[:, :, 0].sum() + (qt * qt).sum() + 1e-3 * (a * a).sum()).backward()
+      torch.cuda.synchronize()
+      out.append([t.grad.cpu().numpy() for t in (q, qd, a)])
+    finally:
+      autotune.set_launch(m, autotune.ADJOINT_PARTS, 0)
+  for x, y in zip(*out):
+    np.testing.assert_array_equal(x, y)
+  assert all(np.isfinite(x).all() for x in out[0])
+  assert autotune.get_launch(m, autotune.ADJOINT_PARTS) == 0
+
+
 def test_autotune_sets_a_candidate():
   sys = S.quadruped(0)
   q0, qd0, acts = _inputs(sys, 512, 2, seed=5)
