@@ -484,6 +484,57 @@ def apg_bench(args):
   }
 
 
+def hetero_cpu_baseline(envs, hb, q0, qd0, acts, K, n_frames=N_FRAMES, chunk=64):
+  """The oracle's C++ restatement on a bounded sample of the mixed batch's
+  timed inputs: its first n_s envs for the K timed env steps, each env
+  stepped with its own System.  Envs of one System run together in chunks
+  of up to `chunk` envs, the chunks on a pool of the host's threads (one
+  OpenMP thread per chunk: most Systems of the mix have a few envs each).
+  Returns the baseline object and the quadrupeds' mean active constraint
+  rows per substep over the sample."""
+  from concurrent.futures import ThreadPoolExecutor
+  from oracle.cstep import CStep
+  threads, host = host_cpu()
+  n_s = timed_sample(len(envs), K)
+  q0n, qd0n, an = q0.cpu().numpy(), qd0.cpu().numpy(), acts[:K].cpu().numpy()
+  by_sys = {}
+  for e in range(n_s):
+    by_sys.setdefault(id(envs[e]), []).append(e)
+  tasks = []
+  for ids in by_sys.values():
+    sys_ = envs[ids[0]]
+    cs = CStep(sys_)
+    for c in range(0, len(ids), chunk):
+      rows = [hb.env_rows(e) for e in ids[c:c + chunk]]
+      q = np.stack([q0n[r[0]] for r in rows])
+      qd = np.stack([qd0n[r[1]] for r in rows])
+      a = np.stack([an[:, r[2]] for r in rows], 1)
+      tasks.append((cs, q, qd, a, sys_.name == "quadruped"))
+  tasks.sort(key=lambda t: -t[1].shape[0] * t[1].shape[1])  # longest first
+  cs0, q, qd, a, _ = tasks[0]
+  cs0.rollout(q[:1], qd[:1], a[:1, :1], n_frames, threads=1)  # load, warm
+
+  def run(t):
+    cs, q, qd, a, quad = t
+    out = cs.rollout(q, qd, a, n_frames, threads=1, rows=quad)
+    return (out[2] * q.shape[0], q.shape[0]) if quad else (0.0, 0)
+
+  t0 = time.perf_counter()
+  with ThreadPoolExecutor(max_workers=threads) as pool:
+    res = list(pool.map(run, tasks))
+  dt = time.perf_counter() - t0
+  n_quad = sum(r[1] for r in res)
+  rows = sum(r[0] for r in res) / n_quad if n_quad else None
+  return {
+      "value": n_s * K / dt, "unit": "env-steps/s", "cores": threads, "kind": "port",
+      "sample": f"the timed region's inputs for its first {n_s} envs ({n_quad} quadrupeds, "
+                f"{n_s - n_quad} chains) x {K} env steps ({n_frames} substeps), "
+                f"oracle/cstep.cpp float32, each env with its own System, {len(tasks)} "
+                f"chunks on {threads} threads, {dt:.1f} s",
+      "host": host,
+  }, rows
+
+
 def hetero_bench(args):
   """Mixed batch: half chains (2-32 links, per-env tables), half quadrupeds,
   in one persistent kernel launch."""
@@ -536,7 +587,13 @@ def hetero_bench(args):
   kernel_ms = ev0.elapsed_time(ev1)
   nonfinite_envs = int(((status & 2) != 0).sum())
   n_quad = n - n_chain
-  r_a = 9.9
+  cpu, r_a = (None, None)
+  if not args.no_cpu:
+    cpu, r_a = hetero_cpu_baseline(envs, hb, q0, qd0, acts, K)
+  r_a_src = ("oracle/cstep.cpp float32 over the CPU baseline's quadrupeds" if r_a is not None
+             else "not measured (--no-cpu): 9.9, the 65536-env mean of earlier runs")
+  if r_a is None:
+    r_a = 9.9
   fl = n_quad * flop_per_substep(quad.num_links(), quad.qd_size(), quad.num_contacts(),
                                  quad.num_limit_rows(), len(quad.geom_type),
                                  quad.solver_iterations, r_a)
@@ -564,8 +621,9 @@ def hetero_bench(args):
       "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                    "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
                    "traffic": None, "kernel_ms": kernel_ms,
-                   "note": "op-count model: quadruped rows at R_a 9.9, chains without rows"},
-      "cpu_baseline": None,
+                   "note": "op-count model: quadrupeds at R_a active rows, chains without rows",
+                   "mean_active_rows": r_a, "mean_active_rows_source": r_a_src},
+      "cpu_baseline": cpu,
       "finite": nonfinite_envs == 0,
       "nonfinite_envs": nonfinite_envs,
   }

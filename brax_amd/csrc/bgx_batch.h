@@ -40,8 +40,19 @@ struct NoTree {
 // n_frames substeps.  An env past the item's count computes on the item's
 // last env and writes nothing.  Status: BGX nonfinite bit per env; probe
 // (homogeneous launches): qf_smooth and qdd of the last substep.
+//
+// Not inlined: in the batch kernel of a tree topology the tree step's
+// register pressure otherwise reaches into the chain loops (spill reloads
+// in the scans, a mixed batch 50% slower than its two halves run apart); as
+// a function of its own the item gets its own register allocation, once
+// per item, and every kernel calls the same code.
+#ifdef BGX_HOST_CHECK
+#define BGX_ITEM inline
+#else
+#define BGX_ITEM __device__ __attribute__((noinline))
+#endif
 template <int G>
-BGX_D void chain_item(const Launch& a, const BatchItem& b, float* lds) {
+BGX_ITEM void chain_item(const Launch& a, const BatchItem& b, float* lds) {
   const int wl = threadIdx.x % WAVE;
   const int l = wl & (G - 1), gb = wl - l, e = wl / G;
   const int L = b.L;
