@@ -101,13 +101,13 @@ def profile_path(kernel, name, n):
   return os.path.join("profiles", f"{kernel}_{name}_{n}.md")
 
 
-def measured_traffic(kernel, name, n, steps):
+def measured_traffic(kernel, name, n, steps, rel=None):
   """HBM bytes of one rollout launch of `steps` env steps of `n` envs, from
   the committed rocprofv3 --pmc summary of that same command (FETCH_SIZE +
   WRITE_SIZE of the rollout kernel, calibrated with a copy kernel), or None
   when no profile of this kernel, workload, batch and step count is
-  committed."""
-  path = os.path.join(ROOT, profile_path(kernel, name, n))
+  committed (rel: the profile's path, if not the kernel's default)."""
+  path = os.path.join(ROOT, rel or profile_path(kernel, name, n))
   if not os.path.exists(path):
     return None
   for line in open(path):
@@ -585,6 +585,12 @@ def hetero_bench(args):
   torch.cuda.synchronize()
   wall = time.perf_counter() - t0
   kernel_ms = ev0.elapsed_time(ev1)
+  if getattr(args, "calib_copy", False):
+    src = torch.ones(64 << 20, device=dev)
+    dst = torch.empty_like(src)
+    torch.add(src, 0.0, out=dst)  # reads 256 MiB, writes 256 MiB
+    torch.cuda.synchronize()
+    del src, dst
   nonfinite_envs = int(((status & 2) != 0).sum())
   n_quad = n - n_chain
   cpu, r_a = (None, None)
@@ -620,7 +626,11 @@ def hetero_bench(args):
                  "parallelism": "none (1 GPU)"},
       "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                    "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
-                   "traffic": None, "kernel_ms": kernel_ms,
+                   "traffic": measured_traffic("k_rollout", "hetero", n, K,
+                                               rel=f"profiles/hetero_{n}.md"),
+                   "traffic_source": f"profiles/hetero_{n}.md (rocprofv3 --pmc FETCH_SIZE, "
+                                     "WRITE_SIZE of this command)",
+                   "kernel_ms": kernel_ms,
                    "note": "op-count model: quadrupeds at R_a active rows, chains without rows",
                    "mean_active_rows": r_a, "mean_active_rows_source": r_a_src},
       "cpu_baseline": cpu,
