@@ -408,17 +408,22 @@ int adj_streams(AdjStreams** out) {
   if (dev < 0 || dev >= kMaxAdjDevices) return fail(BGX_ERR_ARG, "device index out of range");
   if (!g_adj_streams[dev]) {
     AdjStreams h{};
-    for (int i = 0; i < kMaxAdjParts - 1; ++i) {
-      if ((rc = check_hip(hipStreamCreateWithFlags(&h.s[i], hipStreamNonBlocking),
-                          "adjoint stream")))
-        return rc;
-      if ((rc = check_hip(hipEventCreateWithFlags(&h.join[i], hipEventDisableTiming),
-                          "adjoint event")))
-        return rc;
+    for (int i = 0; i < kMaxAdjParts - 1 && !rc; ++i) {
+      rc = check_hip(hipStreamCreateWithFlags(&h.s[i], hipStreamNonBlocking), "adjoint stream");
+      if (!rc)
+        rc = check_hip(hipEventCreateWithFlags(&h.join[i], hipEventDisableTiming),
+                       "adjoint event");
     }
-    if ((rc = check_hip(hipEventCreateWithFlags(&h.fork, hipEventDisableTiming),
-                        "adjoint event")))
+    if (!rc)
+      rc = check_hip(hipEventCreateWithFlags(&h.fork, hipEventDisableTiming), "adjoint event");
+    if (rc) {  // release what was created; the next split call tries again
+      for (int i = 0; i < kMaxAdjParts - 1; ++i) {
+        if (h.s[i]) (void)hipStreamDestroy(h.s[i]);
+        if (h.join[i]) (void)hipEventDestroy(h.join[i]);
+      }
+      if (h.fork) (void)hipEventDestroy(h.fork);
       return rc;
+    }
     g_adj_streams[dev] = new AdjStreams(h);
   }
   *out = g_adj_streams[dev];
