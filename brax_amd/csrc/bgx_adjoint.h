@@ -295,11 +295,12 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
     return vp;
   };
 
-  // J z on the active rows
+  // J z on the active rows (a lane pair splits the slots: sub 0 the even
+  // ones, sub 1 the odd ones, into the pair's shared row vector)
   auto J_apply = [&](const float* z, const SV& out) {
     store_link_vel(z);
 #pragma unroll 2
-    for (int s = 0; s < n_c; ++s) {
+    for (int s = P == 2 ? sub : 0; s < n_c; s += P) {
       V3 w, wa;
       const V3 vp = slot_pvel(s, w, wa);
       const float e0 = dot(slot_v3(s, 4), vp);
@@ -319,10 +320,13 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
     al0 = (x0 + x1) + (x2 + x3), al1 = x0 - x1, al2 = x2 - x3;
     return slot_v3(s, 4) * al0 + slot_v3(s, 7) * al1 + slot_v3(s, 10) * al2;
   };
+  // J^T x: a lane pair splits the slots as J_apply does, each lane projects
+  // the link forces of its own slots, and the pair adds the two (the same
+  // sum on both lanes)
   auto JT_apply = [&](const SV& x, float* tau) {
     clear_link_force();
 #pragma unroll 2
-    for (int s = 0; s < n_c; ++s) {
+    for (int s = P == 2 ? sub : 0; s < n_c; s += P) {
       float a0, a1, a2;
       const V3 w = slot_force(x, s, a0, a1, a2);
       add_link_force(slot_link(s), cross(slot_v3(s, 1), w), w);
@@ -332,6 +336,10 @@ BGX_D void adjoint_rev(const float* __restrict__ M, const float* q, const float*
     V3 fa[L], fv[L];
     load_link_force(fa, fv);
     project_forces<T>(t, fa, fv, tau);
+    if constexpr (P == 2) {
+#pragma unroll
+      for (int d = 0; d < NV; ++d) tau[d] += pair_other(tau[d]);
+    }
 #pragma unroll
     for (int i = 0; i < T::NLIM; ++i) tau[T::LIM_D[i]] += lside[i] * x[NRC + i];
   };
