@@ -358,9 +358,20 @@ __global__ __launch_bounds__(WAVE) void k_adj_fwd(AdjLaunch a) {
   for (int k = 0; k < NV; ++k) qd[k] = qds[(long long)env * NV + k];
 #pragma unroll
   for (int k = 0; k < T::NU; ++k) act[k] = as ? as[(long long)env * T::NU + k] : 0.f;
-  PrivRows<KS> rows;
-  substep_ks<T, KS>(a.M, q, qd, act, smem + threadIdx.x, rows, nullptr,
-                    RecPtr{a.rec + (long long)blockIdx.y * a.rec_stride + env, a.n});
+  // the step's own order: the fast path (KC slots in LDS) records the tape
+  // when the active contacts fit, else the path with a slot per candidate
+  // records it (the tape's layout has a slot per candidate either way)
+  const RecPtr tape{a.rec + (long long)blockIdx.y * a.rec_stride + env, a.n};
+  float* lds = smem + threadIdx.x;
+  bool done = false;
+  if constexpr (T::KC < KS) {
+    LdsRows<T::KC> lrows{lds + T::L * 6 * WAVE};
+    done = substep_ks<T, T::KC>(a.M, q, qd, act, lds, lrows, nullptr, tape);
+  }
+  if (!done) {
+    PrivRows<KS> rows;
+    substep_ks<T, KS>(launder(a.M), q, qd, act, lds, rows, nullptr, tape);
+  }
 }
 
 // adjoint stage A2: the reverse sweep from the tape; its row vectors live in
